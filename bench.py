@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark: time-to-solution of the 33-qubit depth-40 random layered circuit
+on one MI355X, with HBM GB/s of the dominant kernel and the per-gate path
+beside it.
+
+    python bench.py [--steps K] [--warmup W] [--n 33] [--depth 40]
+                    [--mode part|fused64|fused32|pergate] [--k K]
+
+One step = one full simulation of the circuit from |0...0> (state reset
+included) with the chosen path; the state (2^n amplitudes) stays in HBM. The
+per-gate path is timed on a bounded sample of the same circuit (its first
+layers) and its full time-to-solution is extrapolated per gate. Accuracy of
+each path is checked against the FP64 CPU oracle on the same generator at a
+size the oracle finishes in seconds. Rank 0 prints one JSON line last.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from hisvsim_amd import capi  # noqa: E402
+from hisvsim_amd.circuit import gate_amplitudes, random_layered  # noqa: E402
+from hisvsim_amd.partition import greedy_parts  # noqa: E402
+
+METRIC = "33q circuit time-to-solution + HBM GB/s; fused-MFMA vs per-gate at ≤1e-6 infid"
+HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK = 78.6e12
+FP32_MFMA_PEAK = 157.3e12
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_path(s: capi.State, mode: str, gates, parts) -> None:
+    s.reset()
+    if mode == "pergate":
+        s.apply_gates(gates)
+    elif mode == "part":
+        for p in parts:
+            s.apply_part(p, 0)
+    else:
+        fm = "f64" if mode == "fused64" else "f32split"
+        for p in parts:
+            s.apply_fused(p, fm)
+
+
+def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0) -> dict:
+    """Infidelity and max |d amp| of the chosen path against the FP64 oracle."""
+    from oracle.oracle import Oracle
+    gates = random_layered(n, depth, seed=seed)
+    parts = greedy_parts(gates, k)
+    want = np.zeros(2 ** n, complex)
+    want[0] = 1
+    o = Oracle()
+    o.apply_gates(want, gates)
+    with capi.State(n) as s:
+        run_path(s, mode, gates, parts)
+        got = s.read()
+    infid = max(0.0, 1.0 - abs(np.vdot(want, got)) ** 2)
+    return {"n": n, "depth": depth, "infidelity": infid, "max_abs_err": float(np.abs(got - want).max())}
+
+
+def cpu_baseline(n_full: int, depth_full: int, k: int, seconds: float = 20.0) -> dict:
+    """The reference (oracle/_ref) timed on the host cores on a bounded sample:
+    the first layers of the same circuit generator at a smaller n, scaled to
+    the full circuit by amplitude updates."""
+    from oracle.oracle import RefSim
+    if not RefSim.available():
+        return {"value": None, "unit": "s", "cores": None, "kind": "reference",
+                "sample": "oracle/_ref/libref.so not present"}
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 64)
+    ref = RefSim()
+    ref.set_threads(cores)
+    n = min(n_full, 26)
+    layers = 1
+    gates = random_layered(n, layers, seed=0)
+    upd = sum(gate_amplitudes(g, n) for g in gates)
+    t = ref.time_from_zero(n, gates=gates)
+    # scale the sample to ~`seconds` of CPU work
+    layers = max(1, min(depth_full, int(seconds / max(t, 1e-3))))
+    gates = random_layered(n, layers, seed=0)
+    upd = sum(gate_amplitudes(g, n) for g in gates)
+    t = ref.time_from_zero(n, gates=gates)
+    rate = upd / t
+    full_gates = random_layered(n_full, depth_full, seed=0)
+    full_upd = sum(gate_amplitudes(g, n_full) for g in full_gates)
+    return {"value": full_upd / rate, "unit": "s", "cores": cores, "kind": "reference",
+            "sample": f"reference per-gate path (StateVector::apply, OpenMP, {cores} threads), "
+                      f"{n}-qubit {layers}-layer prefix of the same generator ({len(gates)} gates, {t:.1f} s); "
+                      f"{rate:.3e} gate-amplitude updates/s, extrapolated to {n_full}q depth {depth_full}",
+            "gate_amp_updates_per_s": rate}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=33)
+    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--mode", default="part", choices=["part", "fused64", "fused32", "pergate"])
+    ap.add_argument("--k", type=int, default=0, help="part width (default: per mode)")
+    ap.add_argument("--pergate-layers", type=int, default=1, help="layers of the per-gate timing sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU layout is described "
+              f"in DESIGN.md and not run here.", file=sys.stderr)
+        return 2
+    k = args.k or {"part": 10, "fused64": 5, "fused32": 5, "pergate": 1}[args.mode]
+    n, depth = args.n, args.depth
+    gates = random_layered(n, depth, seed=0)
+    parts = greedy_parts(gates, k) if args.mode != "pergate" else []
+    S = 16
+    total_upd = sum(gate_amplitudes(g, n) for g in gates)
+    log(f"[bench] n={n} depth={depth} gates={len(gates)} mode={args.mode} k={k} parts={len(parts)}")
+
+    with capi.State(n) as s:
+        for w in range(args.warmup):
+            run_path(s, args.mode, gates, parts)
+            s.synchronize()
+            log(f"[bench] warmup {w} done")
+        s.set_timing(True)
+        s.kernel_times()
+        s.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            run_path(s, args.mode, gates, parts)
+            log(f"[bench] step {i} queued")
+        s.synchronize()
+        t1 = time.perf_counter()
+        times = s.kernel_times()
+        s.set_timing(False)
+        tts = (t1 - t0) / args.steps
+        # per-launch durations: one per timed call (pass) in each step
+        per_step = len(times) // max(1, args.steps)
+        launches = times[: per_step * args.steps].reshape(args.steps, per_step)
+        kern_ms = float(launches.mean())
+        passes = launches.shape[1]
+        log(f"[bench] TTS {tts:.3f} s, {passes} passes/step, mean pass {kern_ms:.3f} ms")
+
+        # per-gate sample: first layers of the same circuit
+        per_layer = n + n // 2
+        sample = gates[: per_layer * args.pergate_layers]
+        s.reset()
+        s.synchronize()
+        s.set_timing(True)
+        s.kernel_times()
+        s.apply_gates(sample)
+        s.synchronize()
+        gt = s.kernel_times()
+        s.set_timing(False)
+        g_bytes = np.array([2 * S * gate_amplitudes(g, n) for g in sample], float)
+        g_ms = gt[: len(sample)]
+        dense = np.array([g.name == "U3" for g in sample])
+        pergate_tts = float(sum((2 * S * gate_amplitudes(g, n)) for g in gates) /
+                            (g_bytes.sum() / (g_ms.sum() * 1e-3)))
+        pergate_gbs_dense = float(g_bytes[dense].sum() / (g_ms[dense].sum() * 1e-3) / 1e9)
+        pergate_gbs = float(g_bytes.sum() / (g_ms.sum() * 1e-3) / 1e9)
+        log(f"[bench] per-gate sample {len(sample)} gates: {pergate_gbs:.0f} GB/s overall, "
+            f"{pergate_gbs_dense:.0f} GB/s dense U3; extrapolated per-gate TTS {pergate_tts:.2f} s")
+
+    if args.mode in ("part", "pergate"):
+        unit_bytes = 2 * S * 2 ** n
+        achieved = unit_bytes / (kern_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_part" if args.mode == "part" else "k_gate",
+                "per_launch": f"{unit_bytes} B (2 x 16 B x 2^{n}: read + write of the state) / {kern_ms:.3f} ms"}
+    else:
+        kk = max(4, min(7, k))
+        flops = 8 * 2 ** kk * 2 ** n
+        peak = FP64_MFMA_PEAK if args.mode == "fused64" else FP32_MFMA_PEAK
+        achieved = flops / (kern_ms * 1e-3)
+        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None, "kernel": "k_fused",
+                "per_launch": f"8 x 2^k x 2^{n} algorithmic flops (k={kk}) / {kern_ms:.3f} ms",
+                "hbm_gbs": 2 * S * 2 ** n / (kern_ms * 1e-3) / 1e9}
+
+    verify = None if args.no_verify else verify_small(args.mode, k, depth)
+    if verify:
+        log(f"[bench] verify vs FP64 oracle at n={verify['n']}: infidelity {verify['infidelity']:.2e}, "
+            f"max|d| {verify['max_abs_err']:.2e}")
+    cpu = None if args.no_cpu else cpu_baseline(n, depth, k)
+    if cpu:
+        log(f"[bench] cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC, "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": tts * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64" if args.mode != "fused32" else "f32split",
+        "data": "synthetic: seeded random layered circuit (U3 angles ~ U[0,2pi), CZ on random matchings), |0..0> input",
+        "config": {"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, complex128 state "
+                               f"({2 ** n * 16 / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
+                   "mode": args.mode, "k": k, "passes": passes, "partitioner": "greedy acyclic"},
+        "hbm_gbs": 2 * S * 2 ** n / (kern_ms * 1e-3) / 1e9,
+        "gate_amp_updates_per_s": total_upd / tts,
+        "per_gate": {"sample_gates": len(sample), "gbs": pergate_gbs, "gbs_dense_u3": pergate_gbs_dense,
+                     "tts_extrapolated_s": pergate_tts, "passes": len(gates),
+                     "speedup_of_mode": pergate_tts / tts},
+        "roofline": roof,
+        "verify": verify,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,275 @@
+"""ctypes binding of the libhsv C ABI (include/hsv.h).
+
+This is the same boundary a non-Python caller binds; tests and bench.py go
+through it. Every call raises ``HsvError`` on a negative status. There is no
+fallback: if libhsv.so is missing or no GPU is present, the calls fail.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from .circuit import Gate
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhsv.so")
+
+HSV_C128, HSV_C64 = 0, 1
+HSV_FUSED_F64, HSV_FUSED_F32SPLIT = 0, 1
+
+# SvSim::GateType numbering (basic_gates.hpp:12-45), as in hsv.h.
+GATE_TYPES = {
+    "I": 3, "X": 4, "Y": 5, "Z": 6, "H": 7, "S": 8, "T": 9, "RX": 10, "RY": 11, "RZ": 12,
+    "RI": 13, "R1": 14, "RXFRAC": 15, "RYFRAC": 16, "RZFRAC": 17, "RIFRAC": 18, "R1FRAC": 19,
+    "SWAP": 20, "C1": 21, "C2": 22, "U1": 23, "U2": 24, "U3": 25, "RXX": 26, "RYY": 27,
+    "RZZ": 28, "TDG": 29, "SDG": 30, "MATRIX": 64,
+}
+FRAC = {"RXFRAC", "RYFRAC", "RZFRAC", "RIFRAC", "R1FRAC"}
+
+STATUS = {-1: "HSV_ERR_ARG", -2: "HSV_ERR_GATE", -3: "HSV_ERR_ALLOC", -4: "HSV_ERR_DEVICE",
+          -5: "HSV_ERR_LIMIT", -6: "HSV_ERR_UNITARY"}
+
+
+class HsvError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class hsv_gate(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("num_controls", ctypes.c_int32), ("num_targets", ctypes.c_int32),
+                ("controls", ctypes.c_int32 * 5), ("targets", ctypes.c_int32 * 5), ("params", ctypes.c_double * 32)]
+
+
+def to_hsv_gate(g) -> hsv_gate:
+    d = hsv_gate()
+    name = g.name.upper()
+    if name not in GATE_TYPES:
+        raise ValueError(f"unknown gate {g.name}")
+    d.type = GATE_TYPES[name]
+    d.num_controls, d.num_targets = len(g.controls), len(g.targets)
+    if d.num_controls > 5 or d.num_targets > 5:
+        raise ValueError("at most 5 controls and 5 targets")
+    for i, c in enumerate(g.controls):
+        d.controls[i] = int(c)
+    for i, t in enumerate(g.targets):
+        d.targets[i] = int(t)
+    if name in FRAC:
+        d.params[0], d.params[2] = float(g.params[0]), float(g.params[1])
+    else:
+        for i, p in enumerate(g.params):
+            p = complex(p)
+            d.params[2 * i], d.params[2 * i + 1] = p.real, p.imag
+    return d
+
+
+def gate_array(gates: Sequence) -> ctypes.Array:
+    arr = (hsv_gate * max(1, len(gates)))()
+    for i, g in enumerate(gates):
+        arr[i] = to_hsv_gate(g)
+    return arr
+
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_G = ctypes.POINTER(hsv_gate)
+
+# name -> (restype, argtypes); mirrors include/hsv.h
+SIGNATURES = {
+    "hsv_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)]),
+    "hsv_destroy": (ctypes.c_int, [_P]),
+    "hsv_num_qubits": (ctypes.c_int, [_P]),
+    "hsv_dtype": (ctypes.c_int, [_P]),
+    "hsv_apply_gate": (ctypes.c_int, [_P, _G]),
+    "hsv_apply_part": (ctypes.c_int, [_P, _G, ctypes.c_int, ctypes.c_int]),
+    "hsv_apply_fused": (ctypes.c_int, [_P, _G, ctypes.c_int, ctypes.c_int]),
+    "hsv_permute_qubits": (ctypes.c_int, [_P, _I]),
+    "hsv_read": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
+    "hsv_write": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
+    "hsv_reset": (ctypes.c_int, [_P]),
+    "hsv_norm2": (ctypes.c_int, [_P, _D]),
+    "hsv_inner": (ctypes.c_int, [_P, _P, _D]),
+    "hsv_max_abs_diff": (ctypes.c_int, [_P, _P, _D]),
+    "hsv_probability": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
+    "hsv_measure": (ctypes.c_int, [_P, _I, ctypes.c_int, _D, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
+    "hsv_gate_matrix": (ctypes.c_int, [_G, _D, _I]),
+    "hsv_fused_unitary": (ctypes.c_int, [ctypes.c_int, _G, ctypes.c_int, _I, _I, _D, _D]),
+    "hsv_synchronize": (ctypes.c_int, [_P]),
+    "hsv_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "hsv_kernel_times": (ctypes.c_int, [_P, _D, ctypes.c_int, _I]),
+    "hsv_set_opt_slots": (None, [ctypes.c_int]),
+    "hsv_get_opt_slots": (ctypes.c_int, []),
+    "hsv_get_counters": (None, [ctypes.POINTER(ctypes.c_longlong)] * 4),
+    "hsv_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libhsv.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} not built: run `python -m hisvsim_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise HsvError(rc, lib().hsv_last_error().decode())
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+def gate_matrix(g) -> np.ndarray:
+    """The matrix a gate is lowered to (host only)."""
+    out = np.zeros(32)
+    dim = ctypes.c_int(0)
+    d = to_hsv_gate(g)
+    _check(lib().hsv_gate_matrix(ctypes.byref(d), _dptr(out), ctypes.byref(dim)))
+    m = (out[0::2] + 1j * out[1::2])
+    k = dim.value
+    return m[:k * k].reshape(k, k)
+
+
+def fused_unitary(n: int, gates: Sequence):
+    """(qubits, U, unitarity_error) of the fused part (host only)."""
+    arr = gate_array(gates)
+    q = (ctypes.c_int * 8)()
+    k = ctypes.c_int(0)
+    u = np.zeros(2 * 4 ** 7)
+    err = ctypes.c_double(0)
+    _check(lib().hsv_fused_unitary(n, arr, len(gates), q, ctypes.byref(k), _dptr(u), ctypes.byref(err)))
+    dim = 2 ** k.value
+    U = (u[0::2] + 1j * u[1::2])[:dim * dim].reshape(dim, dim)
+    return tuple(q[: k.value]), U, err.value
+
+
+class State:
+    """An n-qubit state in HBM (one libhsv handle)."""
+
+    def __init__(self, n: int, dtype: str = "complex128"):
+        self.n = n
+        self.dtype = dtype
+        self._h = ctypes.c_void_p()
+        dt = {"complex128": HSV_C128, "complex64": HSV_C64}[dtype]
+        _check(lib().hsv_create(n, dt, ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().hsv_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- gates ---------------------------------------------------------------
+    def apply_gate(self, g) -> None:
+        d = to_hsv_gate(g)
+        _check(lib().hsv_apply_gate(self._h, ctypes.byref(d)))
+
+    def apply_gates(self, gates: Iterable) -> None:
+        for g in gates:
+            self.apply_gate(g)
+
+    def apply_part(self, gates: Sequence, opt_slots: int = 1) -> None:
+        arr = gate_array(gates)
+        _check(lib().hsv_apply_part(self._h, arr, len(gates), int(opt_slots)))
+
+    def apply_fused(self, gates: Sequence, mode: str = "f64") -> None:
+        m = {"f64": HSV_FUSED_F64, "f32split": HSV_FUSED_F32SPLIT}[mode]
+        arr = gate_array(gates)
+        _check(lib().hsv_apply_fused(self._h, arr, len(gates), m))
+
+    def permute_qubits(self, new_slot: Sequence[int]) -> None:
+        arr = (ctypes.c_int * self.n)(*[int(x) for x in new_slot])
+        _check(lib().hsv_permute_qubits(self._h, arr))
+
+    # -- data ----------------------------------------------------------------
+    def read(self, offset: int = 0, count: int | None = None) -> np.ndarray:
+        count = (2 ** self.n - offset) if count is None else count
+        out = np.empty(count, dtype=np.complex128)
+        _check(lib().hsv_read(self._h, offset, count, out.ctypes.data_as(_D)))
+        return out
+
+    def write(self, amps: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(amps, dtype=np.complex128)
+        _check(lib().hsv_write(self._h, offset, a.size, a.ctypes.data_as(_D)))
+
+    def reset(self) -> None:
+        _check(lib().hsv_reset(self._h))
+
+    def norm2(self) -> float:
+        v = ctypes.c_double()
+        _check(lib().hsv_norm2(self._h, ctypes.byref(v)))
+        return v.value
+
+    def inner(self, other: "State") -> complex:
+        v = (ctypes.c_double * 2)()
+        _check(lib().hsv_inner(self._h, other._h, v))
+        return complex(v[0], v[1])
+
+    def max_abs_diff(self, other: "State") -> float:
+        v = ctypes.c_double()
+        _check(lib().hsv_max_abs_diff(self._h, other._h, ctypes.byref(v)))
+        return v.value
+
+    def probability(self, qubit: int, value: int = 0) -> float:
+        v = ctypes.c_double()
+        _check(lib().hsv_probability(self._h, qubit, value, ctypes.byref(v)))
+        return v.value
+
+    def measure(self, qubits: Sequence[int], uniforms: np.ndarray) -> np.ndarray:
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        out = np.zeros((len(u), len(qubits)), dtype=np.int32)
+        q = (ctypes.c_int * max(1, len(qubits)))(*qubits)
+        _check(lib().hsv_measure(self._h, q, len(qubits), u.ctypes.data_as(_D), len(u),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return out
+
+    # -- sync / timing ----------------------------------------------------------
+    def synchronize(self) -> None:
+        _check(lib().hsv_synchronize(self._h))
+
+    def set_timing(self, on: bool) -> None:
+        _check(lib().hsv_set_timing(self._h, int(on)))
+
+    def kernel_times(self, capacity: int = 65536) -> np.ndarray:
+        """Durations (ms) of the timed calls since timing was switched on or
+        last collected, in call order."""
+        out = np.zeros(capacity)
+        cnt = ctypes.c_int(0)
+        _check(lib().hsv_kernel_times(self._h, out.ctypes.data_as(_D), capacity, ctypes.byref(cnt)))
+        return out[:min(cnt.value, capacity)]
+
+
+def counters() -> dict:
+    v = [ctypes.c_longlong() for _ in range(4)]
+    lib().hsv_get_counters(*[ctypes.byref(x) for x in v])
+    return dict(zip(("apply_us", "gates", "parts", "part_us"), (x.value for x in v)))

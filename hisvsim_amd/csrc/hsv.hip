@@ -1,0 +1,592 @@
+// libhsv: MI355X (gfx950) hierarchical state-vector library -- C ABI.
+//
+// Host C++ that replaces the reference's state_vector.hpp / loop.hpp /
+// basic_gates.hpp / execute.hpp for one GPU: the state lives in HBM, each gate
+// is lowered on the host (gates.cpp) to a small matrix and applied by one
+// coalesced sweep kernel, and each part runs as one LDS-staged pass.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hsv.h"
+#include "fused.hpp"
+#include "gates.hpp"
+#include "kernels.hpp"
+
+using namespace hsv;
+
+namespace {
+
+thread_local std::string g_err;
+std::atomic<int> g_opt_slots{0};
+std::atomic<long long> g_gate_count{0}, g_part_count{0}, g_part_us{0}, g_apply_us{0};
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(HSV_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct hsv_state {
+  int n = 0;
+  int dtype = HSV_C128;
+  uint64_t len = 0;
+  void* amp = nullptr;
+  hipStream_t stream = nullptr;
+  void* gate_buf = nullptr;        // device copy of the current part's gates
+  size_t gate_buf_bytes = 0;
+  double* partial = nullptr;       // device reduction partials
+  std::vector<double> host_partial;
+  // Event pairs around each timed launch (a pool reused across collections).
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  size_t events_used = 0;
+  int timing = 0;
+  int max_lds = 65536;
+  int num_cu = 256;
+  DeviceScratch fused_scratch;       // fused unitary + offset tables
+  size_t elem() const { return dtype == HSV_C128 ? 16 : 8; }
+};
+
+namespace {
+
+template <typename R>
+typename VecT<R>::type* amp_as(hsv_state* s) {
+  return static_cast<typename VecT<R>::type*>(s->amp);
+}
+
+int begin_timed(hsv_state* s) {
+  if (!s->timing) return HSV_OK;
+  if (s->events_used == s->events.size()) {
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    s->events.emplace_back(a, b);
+  }
+  HIPCHK(hipEventRecord(s->events[s->events_used].first, s->stream));
+  return HSV_OK;
+}
+int end_timed(hsv_state* s) {
+  if (!s->timing) return HSV_OK;
+  HIPCHK(hipEventRecord(s->events[s->events_used].second, s->stream));
+  ++s->events_used;
+  return HSV_OK;
+}
+
+int check_gate_qubits(const hsv_state* s, const Lowered& L) {
+  for (int i = 0; i < L.nt; ++i)
+    if (L.t[i] >= s->n) return fail(HSV_ERR_ARG, "target qubit " + std::to_string(L.t[i]) + " out of range");
+  for (int i = 0; i < L.nc; ++i)
+    if (L.c[i] >= s->n) return fail(HSV_ERR_ARG, "control qubit " + std::to_string(L.c[i]) + " out of range");
+  return HSV_OK;
+}
+
+// Sweep geometry shared by the global and LDS kernels: fixed bit positions
+// (ascending), forced-one mask, target offsets and the work-item count over
+// an index space of `nbits` bits. `map` renames qubits to slots.
+template <typename Map>
+void sweep_geometry(const Lowered& L, int nbits, Map map, int* nfixed, int* fixed, uint64_t* setmask,
+                    uint64_t* off1, uint64_t* off2, uint64_t* nwork) {
+  uint64_t fixedmask = 0, set = 0;
+  for (int i = 0; i < L.nc; ++i) { fixedmask |= 1ull << map(L.c[i]); set |= 1ull << map(L.c[i]); }
+  for (int i = 0; i < L.nt; ++i) fixedmask |= 1ull << map(L.t[i]);
+  if (L.kind == K_PHASE1) set |= 1ull << map(L.t[0]);
+  int nf = 0;
+  for (int b = 0; b < 64; ++b)
+    if (fixedmask >> b & 1) fixed[nf++] = b;
+  for (int i = nf; i < 7; ++i) fixed[i] = 0;
+  *nfixed = nf;
+  *setmask = set;
+  *off1 = 1ull << map(L.t[0]);
+  *off2 = L.nt == 2 ? 1ull << map(L.t[1]) : 0;
+  *nwork = 1ull << (nbits - nf);
+}
+
+void pack_matrix(const Lowered& L, double* m) {
+  for (int i = 0; i < 16; ++i) { m[2 * i] = L.m[i].real(); m[2 * i + 1] = L.m[i].imag(); }
+}
+
+int grid_for(uint64_t items, uint64_t per_block, uint64_t cap) {
+  uint64_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  return (int)std::min<uint64_t>(g, cap);
+}
+
+template <typename R>
+int launch_gate_t(hsv_state* s, const Lowered& L) {
+  GateArgs g{};
+  sweep_geometry(L, s->n, [](int q) { return q; }, &g.nfixed, g.fixed, &g.setmask, &g.off1, &g.off2, &g.nwork);
+  pack_matrix(L, g.m);
+  constexpr int ITEMS = 4;
+  const int grid = grid_for(g.nwork, uint64_t(kGateBlock) * ITEMS, 1u << 20);
+  auto* a = amp_as<R>(s);
+  switch (L.kind) {
+    case K_PHASE1: hipLaunchKernelGGL((k_gate<R, GK_PHASE1, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_DIAG1: hipLaunchKernelGGL((k_gate<R, GK_DIAG1, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_XSWAP1: hipLaunchKernelGGL((k_gate<R, GK_XSWAP1, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_DENSE1: hipLaunchKernelGGL((k_gate<R, GK_DENSE1, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_DIAG2: hipLaunchKernelGGL((k_gate<R, GK_DIAG2, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_SWAP2: hipLaunchKernelGGL((k_gate<R, GK_SWAP2, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_DENSE2: hipLaunchKernelGGL((k_gate<R, GK_DENSE2, 2>), grid_for(g.nwork, uint64_t(kGateBlock) * 2, 1u << 20),
+                                      kGateBlock, 0, s->stream, a, g); break;
+    default: return HSV_OK;
+  }
+  HIPCHK(hipGetLastError());
+  return HSV_OK;
+}
+
+int launch_gate(hsv_state* s, const Lowered& L) {
+  if (L.kind == K_IDENT) return HSV_OK;
+  if (int rc = check_gate_qubits(s, L)) return rc;
+  return s->dtype == HSV_C128 ? launch_gate_t<double>(s, L) : launch_gate_t<float>(s, L);
+}
+
+int max_part_qubits(const hsv_state* s) {
+  int k = 0;
+  while (k < kMaxPartQubits && (size_t(1) << (k + 1)) * s->elem() + 4096 <= (size_t)s->max_lds) ++k;
+  return k;
+}
+
+int ensure_gate_buf(hsv_state* s, size_t bytes) {
+  if (bytes <= s->gate_buf_bytes) return HSV_OK;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->gate_buf) HIPCHK(hipFree(s->gate_buf));
+  s->gate_buf = nullptr;
+  s->gate_buf_bytes = 0;
+  size_t cap = std::max<size_t>(bytes, 1 << 16);
+  if (hipMalloc(&s->gate_buf, cap) != hipSuccess) return fail(HSV_ERR_ALLOC, "gate buffer allocation failed");
+  s->gate_buf_bytes = cap;
+  return HSV_OK;
+}
+
+template <typename R>
+int launch_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t part_mask) {
+  const int k = __builtin_popcountll(part_mask);
+  int inner_of[64];
+  for (int b = 0, r = 0; b < 64; ++b) inner_of[b] = (part_mask >> b & 1) ? r++ : -1;
+  std::vector<PartGate> pg;
+  pg.reserve(Ls.size());
+  for (const Lowered& L : Ls) {
+    if (L.kind == K_IDENT) continue;
+    PartGate G{};
+    uint64_t set, o1, o2, nw;
+    sweep_geometry(L, k, [&](int q) { return inner_of[q]; }, &G.nfixed, G.fixed, &set, &o1, &o2, &nw);
+    G.kind = L.kind;
+    G.setmask = (uint32_t)set;
+    G.off1 = (uint32_t)o1;
+    G.off2 = (uint32_t)o2;
+    G.nwork = (uint32_t)nw;
+    pack_matrix(L, G.m);
+    pg.push_back(G);
+  }
+  const size_t bytes = pg.size() * sizeof(PartGate);
+  if (int rc = ensure_gate_buf(s, bytes)) return rc;
+  if (bytes) HIPCHK(hipMemcpyAsync(s->gate_buf, pg.data(), bytes, hipMemcpyHostToDevice, s->stream));
+  const uint64_t all = s->n == 64 ? ~0ull : ((1ull << s->n) - 1);
+  const uint64_t outer = all & ~part_mask;
+  const uint64_t nbases = 1ull << (s->n - k);
+  const size_t lds = (size_t(1) << k) * sizeof(typename VecT<R>::type);
+  auto* a = amp_as<R>(s);
+  auto* gates = static_cast<const PartGate*>(s->gate_buf);
+  const int ng = (int)pg.size();
+  if (k >= 9) {
+    const int grid = (int)std::min<uint64_t>(nbases, uint64_t(s->num_cu) * 16);
+    if (lds > 65536) HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_part<R, 256>),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_part<R, 256>), grid, 256, lds, s->stream, a, nbases, outer, part_mask, k, gates, ng);
+  } else {
+    const int grid = (int)std::min<uint64_t>(nbases, uint64_t(s->num_cu) * 32);
+    hipLaunchKernelGGL((k_part<R, 64>), grid, 64, lds, s->stream, a, nbases, outer, part_mask, k, gates, ng);
+  }
+  HIPCHK(hipGetLastError());
+  return HSV_OK;
+}
+
+template <typename R, int MODE>
+int reduce_t(hsv_state* a, hsv_state* b, int qubit, double* out0, double* out1) {
+  auto* pa = amp_as<R>(a);
+  auto* pb = b ? amp_as<R>(b) : nullptr;
+  int blocks = kRedBlocks;
+  if (MODE == RD_CHUNK) blocks = (int)std::min<uint64_t>(kRedBlocks, a->len);
+  hipLaunchKernelGGL((k_reduce<R, MODE>), blocks, kRedBlock, 0, a->stream, pa, pb, a->len, qubit, a->partial);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(a->host_partial.data(), a->partial, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost,
+                        a->stream));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  if (MODE == RD_CHUNK) return blocks;
+  double s0 = 0, s1 = 0;
+  for (int i = 0; i < blocks; ++i) {
+    if (MODE == RD_MAXDIFF) s0 = std::max(s0, a->host_partial[2 * i]);
+    else { s0 += a->host_partial[2 * i]; s1 += a->host_partial[2 * i + 1]; }
+  }
+  if (out0) *out0 = s0;
+  if (out1) *out1 = s1;
+  return HSV_OK;
+}
+
+template <int MODE>
+int reduce(hsv_state* a, hsv_state* b, int qubit, double* o0, double* o1) {
+  return a->dtype == HSV_C128 ? reduce_t<double, MODE>(a, b, qubit, o0, o1) : reduce_t<float, MODE>(a, b, qubit, o0, o1);
+}
+
+int lower_all(const hsv_gate* gates, int num_gates, std::vector<Lowered>* out, const hsv_state* s) {
+  out->resize(num_gates);
+  for (int i = 0; i < num_gates; ++i) {
+    std::string err;
+    if (int rc = lower(gates[i], &(*out)[i], &err)) return fail(rc, "gate " + std::to_string(i) + ": " + err);
+    if (int rc = check_gate_qubits(s, (*out)[i])) return rc;
+  }
+  return HSV_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char* hsv_last_error(void) { return g_err.c_str(); }
+
+int hsv_create(int num_qubits, int dtype, hsv_state** out) {
+  if (!out) return fail(HSV_ERR_ARG, "null output handle");
+  *out = nullptr;
+  if (num_qubits < 1 || num_qubits > 36) return fail(HSV_ERR_ARG, "num_qubits must be in [1, 36]");
+  if (dtype != HSV_C128 && dtype != HSV_C64) return fail(HSV_ERR_ARG, "dtype must be HSV_C128 or HSV_C64");
+  hsv_state* s = new hsv_state();
+  s->n = num_qubits;
+  s->dtype = dtype;
+  s->len = 1ull << num_qubits;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  s->max_lds = (int)prop.sharedMemPerBlock;
+  s->num_cu = prop.multiProcessorCount;
+  if (hipStreamCreate(&s->stream) != hipSuccess) { delete s; return fail(HSV_ERR_DEVICE, "hipStreamCreate failed"); }
+  if (hipMalloc(&s->amp, s->len * s->elem()) != hipSuccess) {
+    hipStreamDestroy(s->stream);
+    delete s;
+    return fail(HSV_ERR_ALLOC, "cannot allocate " + std::to_string(s->len * (dtype == HSV_C128 ? 16 : 8)) +
+                                   " bytes of device memory for the state");
+  }
+  if (hipMalloc(reinterpret_cast<void**>(&s->partial), sizeof(double) * 2 * kRedBlocks) != hipSuccess) {
+    hipFree(s->amp);
+    hipStreamDestroy(s->stream);
+    delete s;
+    return fail(HSV_ERR_ALLOC, "cannot allocate reduction scratch");
+  }
+  s->host_partial.resize(2 * kRedBlocks);
+  *out = s;
+  return hsv_reset(s);
+}
+
+int hsv_destroy(hsv_state* s) {
+  if (!s) return HSV_OK;
+  hipStreamSynchronize(s->stream);
+  hipFree(s->amp);
+  hipFree(s->partial);
+  if (s->gate_buf) hipFree(s->gate_buf);
+  if (s->fused_scratch.ptr) hipFree(s->fused_scratch.ptr);
+  for (auto& e : s->events) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  hipStreamDestroy(s->stream);
+  delete s;
+  return HSV_OK;
+}
+
+int hsv_num_qubits(const hsv_state* s) { return s ? s->n : HSV_ERR_ARG; }
+int hsv_dtype(const hsv_state* s) { return s ? s->dtype : HSV_ERR_ARG; }
+
+int hsv_reset(hsv_state* s) {
+  if (!s) return fail(HSV_ERR_ARG, "null handle");
+  const int grid = grid_for(s->len, 256, 1u << 16);
+  if (s->dtype == HSV_C128)
+    hipLaunchKernelGGL((k_zero_state<double>), grid, 256, 0, s->stream, amp_as<double>(s), s->len);
+  else
+    hipLaunchKernelGGL((k_zero_state<float>), grid, 256, 0, s->stream, amp_as<float>(s), s->len);
+  HIPCHK(hipGetLastError());
+  return HSV_OK;
+}
+
+int hsv_apply_gate(hsv_state* s, const hsv_gate* g) {
+  if (!s || !g) return fail(HSV_ERR_ARG, "null argument");
+  Lowered L;
+  std::string err;
+  if (int rc = lower(*g, &L, &err)) return fail(rc, err);
+  if (int rc = begin_timed(s)) return rc;
+  if (int rc = launch_gate(s, L)) return rc;
+  g_gate_count++;
+  return end_timed(s);
+}
+
+int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_slots) {
+  if (!s || (num_gates > 0 && !gates) || num_gates < 0) return fail(HSV_ERR_ARG, "bad arguments");
+  std::vector<Lowered> Ls;
+  if (int rc = lower_all(gates, num_gates, &Ls, s)) return rc;
+  uint64_t part = 0;
+  for (const Lowered& L : Ls) {
+    for (int i = 0; i < L.nt; ++i) part |= 1ull << L.t[i];
+    for (int i = 0; i < L.nc; ++i) part |= 1ull << L.c[i];
+  }
+  if (opt_slots) part |= (s->n >= 4 ? 0xFull : ((1ull << s->n) - 1));  // execute.hpp:184-199
+  const int kmax = max_part_qubits(s);
+  // Slots 0..2 make every gathered run at least 8 consecutive amplitudes; they
+  // join the part whenever the LDS budget allows (results are unchanged).
+  for (int b = 0; b < 3 && b < s->n; ++b)
+    if (!(part >> b & 1) && __builtin_popcountll(part) < kmax) part |= 1ull << b;
+  if (int rc = begin_timed(s)) return rc;
+  if (__builtin_popcountll(part) > kmax) {
+    // Too wide for one workgroup's LDS: apply the part gate by gate.
+    for (const Lowered& L : Ls)
+      if (int rc = launch_gate(s, L)) return rc;
+  } else if (num_gates > 0) {
+    int rc = s->dtype == HSV_C128 ? launch_part_t<double>(s, Ls, part) : launch_part_t<float>(s, Ls, part);
+    if (rc) return rc;
+  }
+  g_part_count++;
+  g_gate_count += num_gates;
+  return end_timed(s);
+}
+
+int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode) {
+  if (!s || (num_gates > 0 && !gates) || num_gates < 0) return fail(HSV_ERR_ARG, "bad arguments");
+  std::vector<Lowered> Ls;
+  if (int rc = lower_all(gates, num_gates, &Ls, s)) return rc;
+  if (int rc = begin_timed(s)) return rc;
+  std::string err;
+  int rc = fused_apply(s->amp, s->n, s->dtype, s->stream, s->num_cu, &s->fused_scratch, Ls, mode, &err);
+  if (rc) return fail(rc, err);
+  g_part_count++;
+  g_gate_count += num_gates;
+  return end_timed(s);
+}
+
+int hsv_permute_qubits(hsv_state* s, const int* new_slot) {
+  if (!s || !new_slot) return fail(HSV_ERR_ARG, "null argument");
+  uint64_t seen = 0;
+  bool identity = true;
+  for (int q = 0; q < s->n; ++q) {
+    if (new_slot[q] < 0 || new_slot[q] >= s->n || (seen >> new_slot[q] & 1))
+      return fail(HSV_ERR_ARG, "new_slot is not a permutation of 0..n-1");
+    seen |= 1ull << new_slot[q];
+    identity = identity && new_slot[q] == q;
+  }
+  if (identity) return HSV_OK;
+  void* out = nullptr;
+  if (hipMalloc(&out, s->len * s->elem()) != hipSuccess)
+    return fail(HSV_ERR_ALLOC, "no room for the permutation's second buffer");
+  int* dslot = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&dslot), sizeof(int) * 64) != hipSuccess) {
+    hipFree(out);
+    return fail(HSV_ERR_ALLOC, "permutation table allocation failed");
+  }
+  HIPCHK(hipMemcpyAsync(dslot, new_slot, sizeof(int) * s->n, hipMemcpyHostToDevice, s->stream));
+  if (int rc = begin_timed(s)) return rc;
+  const int grid = grid_for(s->len, 256, uint64_t(s->num_cu) * 8);
+  if (s->dtype == HSV_C128)
+    hipLaunchKernelGGL((k_permute<double>), grid, 256, 0, s->stream, amp_as<double>(s),
+                       static_cast<double2*>(out), s->len, s->n, dslot);
+  else
+    hipLaunchKernelGGL((k_permute<float>), grid, 256, 0, s->stream, amp_as<float>(s),
+                       static_cast<float2*>(out), s->len, s->n, dslot);
+  HIPCHK(hipGetLastError());
+  if (int rc = end_timed(s)) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipFree(s->amp));
+  HIPCHK(hipFree(dslot));
+  s->amp = out;
+  return HSV_OK;
+}
+
+int hsv_read(hsv_state* s, uint64_t offset, uint64_t count, double* host_out) {
+  if (!s || (count && !host_out) || offset > s->len || count > s->len - offset)
+    return fail(HSV_ERR_ARG, "read range out of bounds");
+  if (!count) return HSV_OK;
+  if (s->dtype == HSV_C128) {
+    HIPCHK(hipMemcpyAsync(host_out, static_cast<char*>(s->amp) + offset * 16, count * 16, hipMemcpyDeviceToHost,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  } else {
+    std::vector<float> tmp(2 * count);
+    HIPCHK(hipMemcpyAsync(tmp.data(), static_cast<char*>(s->amp) + offset * 8, count * 8, hipMemcpyDeviceToHost,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (uint64_t i = 0; i < 2 * count; ++i) host_out[i] = tmp[i];
+  }
+  return HSV_OK;
+}
+
+int hsv_write(hsv_state* s, uint64_t offset, uint64_t count, const double* host_in) {
+  if (!s || (count && !host_in) || offset > s->len || count > s->len - offset)
+    return fail(HSV_ERR_ARG, "write range out of bounds");
+  if (!count) return HSV_OK;
+  if (s->dtype == HSV_C128) {
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(s->amp) + offset * 16, host_in, count * 16, hipMemcpyHostToDevice,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  } else {
+    std::vector<float> tmp(2 * count);
+    for (uint64_t i = 0; i < 2 * count; ++i) tmp[i] = (float)host_in[i];
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(s->amp) + offset * 8, tmp.data(), count * 8, hipMemcpyHostToDevice,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  return HSV_OK;
+}
+
+int hsv_norm2(hsv_state* s, double* out) {
+  if (!s || !out) return fail(HSV_ERR_ARG, "null argument");
+  return reduce<RD_NORM2>(s, nullptr, 0, out, nullptr);
+}
+
+int hsv_inner(hsv_state* a, hsv_state* b, double* out2) {
+  if (!a || !b || !out2) return fail(HSV_ERR_ARG, "null argument");
+  if (a->n != b->n || a->dtype != b->dtype) return fail(HSV_ERR_ARG, "states differ in size or dtype");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return reduce<RD_INNER>(a, b, 0, &out2[0], &out2[1]);
+}
+
+int hsv_max_abs_diff(hsv_state* a, hsv_state* b, double* out) {
+  if (!a || !b || !out) return fail(HSV_ERR_ARG, "null argument");
+  if (a->n != b->n || a->dtype != b->dtype) return fail(HSV_ERR_ARG, "states differ in size or dtype");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return reduce<RD_MAXDIFF>(a, b, 0, out, nullptr);
+}
+
+int hsv_probability(hsv_state* s, int qubit, int value, double* out) {
+  if (!s || !out || qubit < 0 || qubit >= s->n || (value != 0 && value != 1))
+    return fail(HSV_ERR_ARG, "bad qubit or value");
+  double p0 = 0;
+  if (int rc = reduce<RD_PROB0>(s, nullptr, qubit, &p0, nullptr)) return rc;
+  *out = value == 0 ? p0 : 1.0 - p0;
+  return HSV_OK;
+}
+
+int hsv_measure(hsv_state* s, const int* qubits, int num_qubits, const double* uniforms, int num_samples,
+                int32_t* out) {
+  if (!s || num_qubits < 0 || num_samples < 0 || (num_qubits && !qubits) || (num_samples && (!uniforms || !out)))
+    return fail(HSV_ERR_ARG, "bad arguments");
+  for (int j = 0; j < num_qubits; ++j)
+    if (qubits[j] < 0 || qubits[j] >= s->n) return fail(HSV_ERR_ARG, "measured qubit out of range");
+  if (!num_samples) return HSV_OK;
+  std::vector<double> u(uniforms, uniforms + num_samples);
+  std::sort(u.begin(), u.end());
+  // Probability mass per contiguous chunk, then the reference's sequential
+  // walk (state_vector.hpp:946-972) inside the chunks the samples land in.
+  int blocks = s->dtype == HSV_C128 ? reduce_t<double, RD_CHUNK>(s, nullptr, 0, nullptr, nullptr)
+                                    : reduce_t<float, RD_CHUNK>(s, nullptr, 0, nullptr, nullptr);
+  if (blocks < 0) return blocks;
+  const uint64_t chunk = s->len / blocks;
+  std::vector<double> amps(2 * chunk);
+  double cum = 0;
+  int sp = 0;
+  for (int c = 0; c < blocks && sp < num_samples; ++c) {
+    const double mass = s->host_partial[2 * c];
+    if (!(u[sp] < cum + mass)) { cum += mass; continue; }
+    if (int rc = hsv_read(s, uint64_t(c) * chunk, chunk, amps.data())) return rc;
+    for (uint64_t i = 0; i < chunk && sp < num_samples; ++i) {
+      const double x = amps[2 * i], y = amps[2 * i + 1];
+      const double p = x * x + y * y;
+      const uint64_t pos = uint64_t(c) * chunk + i;
+      while (sp < num_samples && u[sp] < cum + p) {
+        for (int j = 0; j < num_qubits; ++j) out[sp * num_qubits + j] = (int32_t)((pos >> qubits[j]) & 1);
+        ++sp;
+      }
+      cum += p;
+    }
+  }
+  for (; sp < num_samples; ++sp)  // mass exhausted: the reference reports all ones (:973-980)
+    for (int j = 0; j < num_qubits; ++j) out[sp * num_qubits + j] = 1;
+  return HSV_OK;
+}
+
+int hsv_gate_matrix(const hsv_gate* g, double* out, int* dim) {
+  if (!g || !out || !dim) return fail(HSV_ERR_ARG, "null argument");
+  Lowered L;
+  std::string err;
+  if (int rc = lower(*g, &L, &err)) return fail(rc, err);
+  cd m[16];
+  dense_of(L, m);
+  *dim = L.nt == 2 ? 4 : 2;
+  for (int i = 0; i < 16; ++i) { out[2 * i] = m[i].real(); out[2 * i + 1] = m[i].imag(); }
+  if (*dim == 2) {  // dense_of leaves a 2x2 in the first four entries
+    for (int i = 4; i < 16; ++i) out[2 * i] = out[2 * i + 1] = 0.0;
+  }
+  return HSV_OK;
+}
+
+int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int* qubits_out, int* num_part_qubits,
+                      double* u_out, double* unitarity_err) {
+  if ((num_gates > 0 && !gates) || !qubits_out || !num_part_qubits || !u_out) return fail(HSV_ERR_ARG, "null argument");
+  std::vector<Lowered> Ls(num_gates);
+  for (int i = 0; i < num_gates; ++i) {
+    std::string err;
+    if (int rc = lower(gates[i], &Ls[i], &err)) return fail(rc, err);
+    for (int j = 0; j < Ls[i].nt; ++j)
+      if (Ls[i].t[j] >= num_qubits) return fail(HSV_ERR_ARG, "qubit out of range");
+    for (int j = 0; j < Ls[i].nc; ++j)
+      if (Ls[i].c[j] >= num_qubits) return fail(HSV_ERR_ARG, "qubit out of range");
+  }
+  std::vector<int> Q;
+  std::vector<cd> U;
+  std::string err;
+  if (int rc = fused_unitary(num_qubits, Ls, &Q, &U, &err)) return fail(rc, err);
+  *num_part_qubits = (int)Q.size();
+  for (size_t i = 0; i < Q.size(); ++i) qubits_out[i] = Q[i];
+  for (size_t i = 0; i < U.size(); ++i) { u_out[2 * i] = U[i].real(); u_out[2 * i + 1] = U[i].imag(); }
+  if (unitarity_err) *unitarity_err = unitarity_error(U, 1 << Q.size());
+  return HSV_OK;
+}
+
+int hsv_synchronize(hsv_state* s) {
+  if (!s) return fail(HSV_ERR_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return HSV_OK;
+}
+
+int hsv_set_timing(hsv_state* s, int on) {
+  if (!s) return fail(HSV_ERR_ARG, "null handle");
+  s->timing = on ? 1 : 0;
+  s->events_used = 0;
+  return HSV_OK;
+}
+
+int hsv_kernel_times(hsv_state* s, double* ms_out, int capacity, int* count) {
+  if (!s || !count || (capacity > 0 && !ms_out)) return fail(HSV_ERR_ARG, "null argument");
+  const int n = (int)s->events_used;
+  *count = n;
+  for (int i = 0; i < n && i < capacity; ++i) {
+    HIPCHK(hipEventSynchronize(s->events[i].second));
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, s->events[i].first, s->events[i].second));
+    ms_out[i] = f;
+    g_part_us += (long long)(f * 1000.0f);
+  }
+  s->events_used = 0;
+  return HSV_OK;
+}
+
+void hsv_set_opt_slots(int on) { g_opt_slots = on ? 1 : 0; }
+int hsv_get_opt_slots(void) { return g_opt_slots; }
+
+void hsv_get_counters(long long* apply_us, long long* gates, long long* parts, long long* part_us) {
+  if (apply_us) *apply_us = g_apply_us;
+  if (gates) *gates = g_gate_count;
+  if (parts) *parts = g_part_count;
+  if (part_us) *part_us = g_part_us;
+}
+
+}  // extern "C"

@@ -1,0 +1,223 @@
+"""Acyclic partitioners and the parts-file format.
+
+The reference takes its parts from an external acyclic DAG partitioner (dagP,
+not vendored) and reads them from a parts file with one line per DAG node,
+``"<order> <node> <part>"`` (qasm_assembler_standalone.py:3549-3567); parts run
+in ascending part id, gates within a part in file order, order 0 (the input
+node) and the exit nodes are skipped (:1783-1791).
+
+Partitioners here (all return a list of parts, each a list of gates, in
+execution order; every part touches at most k qubits and the sequence is a
+valid topological order of the gate DAG):
+
+* ``nat_parts`` -- natural-order greedy: start a new part when the next gate
+  would push the part past k qubits. Reproduces the shipped ``*_part_nat``.
+* ``greedy_parts`` -- DAG-aware greedy: fills each part by scanning the
+  remaining gates in order, taking every gate whose qubits fit and whose
+  predecessors are all placed; a skipped gate blocks its qubits for the rest
+  of the scan, which keeps the quotient DAG acyclic.
+* ``layered_parts`` -- for circuits made of single-qubit layers and two-qubit
+  matchings (the random layered benchmark): groups each matching's pairs so a
+  part holds whole pairs plus the single-qubit gates that follow them.
+"""
+from __future__ import annotations
+
+from collections import OrderedDict
+from typing import Dict, List, Sequence
+
+from .circuit import Gate
+
+
+def gate_qubits(g: Gate) -> set:
+    return set(g.controls) | set(g.targets)
+
+
+def nat_parts(gates: Sequence[Gate], k: int) -> List[List[Gate]]:
+    parts: List[List[Gate]] = []
+    cur: List[Gate] = []
+    qs: set = set()
+    for g in gates:
+        gq = gate_qubits(g)
+        if cur and len(qs | gq) > k:
+            parts.append(cur)
+            cur, qs = [], set()
+        cur.append(g)
+        qs |= gq
+    if cur:
+        parts.append(cur)
+    return parts
+
+
+def nat_part_ids(gates: Sequence[Gate], k: int) -> List[int]:
+    """Part id of every gate under ``nat_parts`` (ids start at 0)."""
+    ids, pid, qs = [], 0, set()
+    for g in gates:
+        gq = gate_qubits(g)
+        if ids and len(qs | gq) > k:
+            pid += 1
+            qs = set()
+        qs |= gq
+        ids.append(pid)
+    return ids
+
+
+def greedy_parts(gates: Sequence[Gate], k: int) -> List[List[Gate]]:
+    remaining = list(range(len(gates)))
+    parts: List[List[Gate]] = []
+    while remaining:
+        part_q: set = set()
+        blocked: set = set()
+        taken: List[int] = []
+        keep: List[int] = []
+        for i in remaining:
+            gq = gate_qubits(gates[i])
+            if gq & blocked:
+                blocked |= gq
+                keep.append(i)
+            elif len(part_q | gq) <= k:
+                part_q |= gq
+                taken.append(i)
+            else:
+                blocked |= gq
+                keep.append(i)
+        parts.append([gates[i] for i in taken])
+        remaining = keep
+    return parts
+
+
+def layered_parts(gates: Sequence[Gate], n: int, k: int) -> List[List[Gate]]:
+    """Parts for layer-structured circuits: each layer is single-qubit gates on
+    some qubits followed by two-qubit gates on disjoint pairs.
+
+    The circuit is cut into segments at the start of every two-qubit block;
+    segment L holds the pairs of block L and the single-qubit gates after them
+    (up to the next block). Pairs are grouped k//2 at a time (an idle qubit
+    fills a slot if k is odd or a group is short), and each group's part holds
+    its pairs plus the following single-qubit gates on its qubits. Gates
+    before the first block are merged into the first segment's parts.
+    Falls back to ``greedy_parts`` if the circuit does not have this shape.
+    """
+    # split into alternating runs of single-qubit and two-qubit gates
+    runs: List[tuple] = []
+    for g in gates:
+        two = len(gate_qubits(g)) == 2
+        if runs and runs[-1][0] == two:
+            runs[-1][1].append(g)
+        else:
+            runs.append((two, [g]))
+    if any(len(gate_qubits(g)) > 2 for g in gates):
+        return greedy_parts(gates, k)
+    for two, rg in runs:
+        if two:
+            seen: set = set()
+            for g in rg:
+                if gate_qubits(g) & seen:
+                    return greedy_parts(gates, k)
+                seen |= gate_qubits(g)
+    pre: List[Gate] = []
+    segments: List[tuple] = []  # (pairs gates, following single-qubit gates)
+    i = 0
+    if runs and not runs[0][0]:
+        pre = runs[0][1]
+        i = 1
+    while i < len(runs):
+        pairs = runs[i][1]
+        singles = runs[i + 1][1] if i + 1 < len(runs) else []
+        segments.append((pairs, singles))
+        i += 2
+    if not segments:
+        return greedy_parts(gates, k)
+    parts: List[List[Gate]] = []
+    for si, (pairs, singles) in enumerate(segments):
+        paired = set()
+        for g in pairs:
+            paired |= gate_qubits(g)
+        idle = [q for q in range(n) if q not in paired]
+        per = max(1, k // 2)
+        groups: List[tuple] = []
+        for j in range(0, len(pairs), per):
+            groups.append((pairs[j:j + per], set().union(*[gate_qubits(g) for g in pairs[j:j + per]])))
+        # idle qubits join groups with room, else form their own
+        for q in idle:
+            for gi, (pg, qs) in enumerate(groups):
+                if len(qs) < k:
+                    qs.add(q)
+                    break
+            else:
+                groups.append(([], {q}))
+        by_qubit: Dict[int, List[Gate]] = {}
+        for g in singles:
+            by_qubit.setdefault(g.targets[0], []).append(g)
+        pre_by_q: Dict[int, List[Gate]] = {}
+        if si == 0:
+            for g in pre:
+                pre_by_q.setdefault(g.targets[0], []).append(g)
+        for pg, qs in groups:
+            part: List[Gate] = []
+            for q in sorted(qs):
+                part += pre_by_q.get(q, [])
+            part += pg
+            for q in sorted(qs):
+                part += by_qubit.get(q, [])
+            if part:
+                parts.append(part)
+    return parts
+
+
+def check_parts(gates: Sequence[Gate], parts: Sequence[Sequence[Gate]], k: int | None = None) -> None:
+    """Raises ValueError unless ``parts`` is a k-bounded acyclic partition of
+    ``gates`` whose sequence respects every qubit's gate order."""
+    pos = {}
+    for i, g in enumerate(gates):
+        pos.setdefault(g, []).append(i)
+    flat = [g for p in parts for g in p]
+    if len(flat) != len(gates):
+        raise ValueError("parts do not cover the gates exactly once")
+    # each qubit's gate sequence must be unchanged
+    def per_qubit(seq):
+        d: Dict[int, list] = {}
+        for g in seq:
+            for q in gate_qubits(g):
+                d.setdefault(q, []).append(g)
+        return d
+    if per_qubit(gates) != per_qubit(flat):
+        raise ValueError("parts reorder gates on some qubit")
+    if k is not None:
+        for p in parts:
+            if len(set().union(*[gate_qubits(g) for g in p])) > k:
+                raise ValueError("a part exceeds k qubits")
+
+
+# ---- parts-file format (qasm_assembler_standalone.py:3549-3567) -------------
+def read_parts_file(path: str) -> "OrderedDict[int, List[int]]":
+    """part id -> [order ids] in file order, parts sorted by id."""
+    parts: Dict[int, List[int]] = {}
+    with open(path) as f:
+        for line in f:
+            line = line.rstrip("\n")
+            if not line:
+                continue
+            items = line.split(" ")
+            if len(items) != 3:
+                raise ValueError(f"bad parts-file line {line!r}")
+            order_id, part_id = int(items[0]), int(items[2])
+            parts.setdefault(part_id, []).append(order_id)
+    return OrderedDict(sorted(parts.items()))
+
+
+def parts_from_file(path: str, ordered_gates: Sequence[Gate]) -> List[List[Gate]]:
+    """Gate lists per part: order 0 (input node) and order ids past the gate
+    list (exit nodes) are skipped; order o maps to ordered_gates[o - 1]."""
+    out = []
+    for _, orders in read_parts_file(path).items():
+        part = [ordered_gates[o - 1] for o in orders if 0 < o <= len(ordered_gates)]
+        if part:
+            out.append(part)
+    return out
+
+
+def write_parts_file(path: str, node_names: Sequence[str], part_ids: Sequence[int]) -> None:
+    """One ``"<order> <node> <part>"`` line per gate (orders from 1)."""
+    with open(path, "w") as f:
+        for i, (name, pid) in enumerate(zip(node_names, part_ids)):
+            f.write(f"{i + 1} {name} {pid}\n")

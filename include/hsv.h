@@ -1,0 +1,157 @@
+/* hsv.h -- C ABI of the MI355X hierarchical state-vector library (libhsv.so).
+ *
+ * Plain C: pointers, sizes and status codes only. Every function returns 0 on
+ * success or a negative hsv_status; hsv_last_error() gives the message of the
+ * last failure on the calling thread. A handle owns its device memory and one
+ * HIP stream; calls on one handle are not thread-safe. Host buffers belong to
+ * the caller. Amplitudes are interleaved (re, im) complex128 (or complex64),
+ * and qubit q is bit q of the amplitude index (qubit 0 = least significant),
+ * as in the reference (state_vector.hpp:60-61, 1017-1023).
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * the HiSVSIM sources).
+ */
+#ifndef HSV_H_
+#define HSV_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hsv_state hsv_state;
+
+enum hsv_status {
+  HSV_OK = 0,
+  HSV_ERR_ARG = -1,      /* bad argument (qubit out of range, duplicate, count) */
+  HSV_ERR_GATE = -2,     /* unknown gate type or wrong number of targets */
+  HSV_ERR_ALLOC = -3,    /* device or host allocation failed */
+  HSV_ERR_DEVICE = -4,   /* HIP runtime error */
+  HSV_ERR_LIMIT = -5,    /* part too wide for the selected kernel */
+  HSV_ERR_UNITARY = -6   /* fused matrix failed its unitarity check */
+};
+
+enum hsv_dtype { HSV_C128 = 0, HSV_C64 = 1 };
+
+/* Gate type numbering mirrors SvSim::GateType (basic_gates.hpp:12-45). */
+enum hsv_gate_type {
+  HSV_G_I = 3, HSV_G_X = 4, HSV_G_Y = 5, HSV_G_Z = 6, HSV_G_H = 7, HSV_G_S = 8,
+  HSV_G_T = 9, HSV_G_RX = 10, HSV_G_RY = 11, HSV_G_RZ = 12, HSV_G_RI = 13,
+  HSV_G_R1 = 14, HSV_G_RXFRAC = 15, HSV_G_RYFRAC = 16, HSV_G_RZFRAC = 17,
+  HSV_G_RIFRAC = 18, HSV_G_R1FRAC = 19, HSV_G_SWAP = 20, HSV_G_C1 = 21,
+  HSV_G_C2 = 22, HSV_G_U1 = 23, HSV_G_U2 = 24, HSV_G_U3 = 25, HSV_G_RXX = 26,
+  HSV_G_RYY = 27, HSV_G_RZZ = 28, HSV_G_TDG = 29, HSV_G_SDG = 30,
+  /* Not in the reference: a caller-supplied dense 2x2 or 4x4 matrix in params
+   * (row-major, index = 2*bit(targets[0]) + bit(targets[1])). */
+  HSV_G_MATRIX = 64
+};
+
+/* One bound gate: the reference's BoundGate (state_vector.hpp:1332-1379)
+ * holding an AggregateGateFactory (basic_gates.hpp:1779-1845). params holds
+ * the complex parameters interleaved (re, im) in constructor order; the
+ * *Frac gates take params[0] = numerator and params[2] = power. At most 5
+ * controls and 5 targets (types.hpp:8-9). */
+typedef struct hsv_gate {
+  int32_t type;
+  int32_t num_controls;
+  int32_t num_targets;
+  int32_t controls[5];
+  int32_t targets[5];
+  double params[32];
+} hsv_gate;
+
+/* Fused-part precision modes. */
+enum hsv_fused_mode {
+  HSV_FUSED_F64 = 0,        /* FP64 matrix cores (v_mfma_f64)            */
+  HSV_FUSED_F32SPLIT = 1    /* FP32 matrix cores, hi/lo split products   */
+};
+
+/* Replaces StateVector(Qubit, int alloc) (state_vector.hpp:60-61, 74-119,
+ * allocate 178-248): 2^n amplitudes in HBM set to |0...0>. The reference's
+ * NUMA policy argument has no device meaning and is not taken. */
+int hsv_create(int num_qubits, int dtype, hsv_state** out);
+/* Frees the state (the reference never frees; state_vector.hpp:173-176). */
+int hsv_destroy(hsv_state* s);
+
+int hsv_num_qubits(const hsv_state* s);
+int hsv_dtype(const hsv_state* s);
+
+/* Replaces BoundGate::apply -> apply_gate -> StateVector::apply
+ * (state_vector.hpp:1349, 1202-1320, 402-510): one pass over the state. */
+int hsv_apply_gate(hsv_state* s, const hsv_gate* g);
+
+/* Replaces gather_and_execute_on (py_wrapper.cpp:156 -> execute.hpp:171-316,
+ * with the live gather/execute/scatter body of execute.hpp:318-470): all
+ * gates of one part applied in one pass, the part's sub-state staged in LDS.
+ * opt_slots adds qubits 0..3 to the part (execute.hpp:184-199). */
+int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_slots);
+
+/* Not in the reference: the part's gates fused on the host into one
+ * 2^k x 2^k FP64 unitary (checked for unitarity) and applied as a batched
+ * complex GEMM on the matrix cores. k <= 7. */
+int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode);
+
+/* Replaces the qubit exchange of SvSimMPI::gather_qubits / gather_slots
+ * (svsim-mpi.hpp:123-173, mpi_redistributer.hpp:265-344) on one device:
+ * amplitude bit s moves to bit new_slot[s]. */
+int hsv_permute_qubits(hsv_state* s, const int* new_slot);
+
+/* Amplitude access (the reference has none beyond dump_state,
+ * state_vector.hpp:128-140). count amplitudes from offset, interleaved
+ * (re, im) doubles on the host, whatever the state's dtype. */
+int hsv_read(hsv_state* s, uint64_t offset, uint64_t count, double* host_out);
+int hsv_write(hsv_state* s, uint64_t offset, uint64_t count, const double* host_in);
+/* Sets |0...0> again. */
+int hsv_reset(hsv_state* s);
+
+/* FP64 reductions. */
+int hsv_norm2(hsv_state* s, double* out);
+/* <a|b> as (re, im). */
+int hsv_inner(hsv_state* a, hsv_state* b, double* out2);
+/* max |a_i - b_i|; replaces the amplitude loop of operator== and state_equal
+ * (state_vector.hpp:1003-1010, 1053-1072; threshold 1e-10, types.hpp:10). */
+int hsv_max_abs_diff(hsv_state* a, hsv_state* b, double* out);
+/* Replaces StateVector::probability (state_vector.hpp:897-923). */
+int hsv_probability(hsv_state* s, int qubit, int value, double* out);
+/* Replaces StateVector::measure (state_vector.hpp:925-981): the same walk
+ * over cumulative probabilities for the given uniforms (sorted here).
+ * out[i*num_qubits + j] = bit of qubits[j] in sample i. */
+int hsv_measure(hsv_state* s, const int* qubits, int num_qubits,
+                const double* uniforms, int num_samples, int32_t* out);
+
+/* Host-only introspection (no device needed; not in the reference).
+ * hsv_gate_matrix: the 2x2 or 4x4 matrix a gate is lowered to, row-major
+ * interleaved (re, im) into out (32 doubles), its size in *dim.
+ * hsv_fused_unitary: the FP64 unitary hsv_apply_fused would apply for these
+ * gates on an n-qubit state: qubits_out (<= 7 entries, sorted) and U row-major
+ * interleaved into u_out (2 * 4^7 doubles); returns its unitarity error. */
+int hsv_gate_matrix(const hsv_gate* g, double* out, int* dim);
+int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int* qubits_out,
+                      int* num_part_qubits, double* u_out, double* unitarity_err);
+
+/* Waits for all work queued on the handle. */
+int hsv_synchronize(hsv_state* s);
+/* Launch timing: while on, every apply/permute call records a HIP event pair
+ * around its kernels on the handle's stream. hsv_kernel_times waits for them,
+ * writes up to `capacity` durations (ms, in call order), sets *count to the
+ * number recorded, and starts a new collection. */
+int hsv_set_timing(hsv_state* s, int on);
+int hsv_kernel_times(hsv_state* s, double* ms_out, int capacity, int* count);
+
+/* Replaces the knobs and counters of py_wrapper.cpp:163-175
+ * (set_opt_slots execute.hpp:35-37; obtain_* state_vector.hpp:1381-1391,
+ * execute.hpp:31-33). Thread counts have no device meaning and are kept
+ * only as values the caller can read back. */
+void hsv_set_opt_slots(int on);
+int hsv_get_opt_slots(void);
+void hsv_get_counters(long long* apply_us, long long* gates, long long* parts,
+                      long long* part_us);
+
+const char* hsv_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HSV_H_ */

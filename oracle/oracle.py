@@ -1,0 +1,246 @@
+"""CPU oracle for the hierarchical state-vector hot path (TEST INFRASTRUCTURE).
+
+Only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may
+import this module, and only as the checker or the timed CPU baseline. The
+product (``hisvsim_amd``) never imports it.
+
+Two back ends share one interface:
+
+* ``Oracle``  -- ``liboracle.so`` built from ``hsv_oracle.cpp``: a from-scratch
+  restatement of the reference arithmetic (basic_gates.hpp:99-1408,
+  state_vector.hpp:402-510, execute.hpp:318-470).
+* ``RefSim`` -- ``_ref/libref.so``: the reference's own headers compiled behind
+  a small C harness (``ref_harness.cpp``). Present only where it was built.
+
+Gates are ``Gate(name, targets, controls, params)`` with reference names
+(basic_gates.hpp:1487-1522, matched case-insensitively) and complex params
+(``AggregateGate_Frac`` takes ``(numerator, power)``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import NamedTuple, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# Numbering of SvSim::GateType (basic_gates.hpp:12-45).
+GATE_TYPES = {
+    "ALLOC": 1, "FREE": 2, "I": 3, "X": 4, "Y": 5, "Z": 6, "H": 7, "S": 8, "T": 9,
+    "RX": 10, "RY": 11, "RZ": 12, "RI": 13, "R1": 14, "RXFRAC": 15, "RYFRAC": 16,
+    "RZFRAC": 17, "RIFRAC": 18, "R1FRAC": 19, "SWAP": 20, "C1": 21, "C2": 22,
+    "U1": 23, "U2": 24, "U3": 25, "RXX": 26, "RYY": 27, "RZZ": 28, "TDG": 29, "SDG": 30,
+}
+FRAC = {"RXFRAC", "RYFRAC", "RZFRAC", "RIFRAC", "R1FRAC"}
+TWO_TARGET = {"SWAP", "C2", "RXX", "RYY", "RZZ"}
+
+
+class Gate(NamedTuple):
+    name: str
+    targets: tuple
+    controls: tuple = ()
+    params: tuple = ()
+
+
+class GateDesc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("nc", ctypes.c_int32), ("nt", ctypes.c_int32),
+                ("controls", ctypes.c_int32 * 5), ("targets", ctypes.c_int32 * 5),
+                ("params", ctypes.c_double * 32)]
+
+
+def to_desc(g: Gate) -> GateDesc:
+    d = GateDesc()
+    name = g.name.upper()
+    d.type = GATE_TYPES[name]
+    d.nc, d.nt = len(g.controls), len(g.targets)
+    for i, c in enumerate(g.controls):
+        d.controls[i] = int(c)
+    for i, t in enumerate(g.targets):
+        d.targets[i] = int(t)
+    if name in FRAC:
+        d.params[0], d.params[2] = float(g.params[0]), float(g.params[1])
+    else:
+        for i, p in enumerate(g.params):
+            p = complex(p)
+            d.params[2 * i], d.params[2 * i + 1] = p.real, p.imag
+    return d
+
+
+def desc_array(gates: Sequence[Gate]):
+    arr = (GateDesc * max(1, len(gates)))()
+    for i, g in enumerate(gates):
+        arr[i] = to_desc(g)
+    return arr
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check_state(state: np.ndarray) -> int:
+    if state.dtype != np.complex128 or not state.flags.c_contiguous:
+        raise TypeError("state must be a C-contiguous complex128 array")
+    n = int(state.size).bit_length() - 1
+    if state.size != 1 << n:
+        raise ValueError("state length must be a power of two")
+    return n
+
+
+class Oracle:
+    """ctypes front end of liboracle.so (the restatement)."""
+
+    def __init__(self, path: str | None = None):
+        path = path or os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        L = self.lib = ctypes.CDLL(path)
+        L.orc_apply_gate.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.POINTER(GateDesc)]
+        L.orc_run_part.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.POINTER(GateDesc),
+                                   ctypes.c_int, ctypes.c_int]
+        L.orc_permute.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.orc_exchange_order.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                         ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.orc_norm2.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.orc_norm2.restype = ctypes.c_double
+        L.orc_probability.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_probability.restype = ctypes.c_double
+        L.orc_measure_sorted.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int32)]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+
+    def set_threads(self, n: int):
+        self.lib.orc_set_threads(int(n))
+
+    def apply_gate(self, state: np.ndarray, g: Gate) -> None:
+        n = _check_state(state)
+        d = to_desc(g)
+        rc = self.lib.orc_apply_gate(_ptr(state), n, ctypes.byref(d))
+        if rc:
+            raise ValueError(f"oracle rejected gate {g} (code {rc})")
+
+    def apply_gates(self, state: np.ndarray, gates: Sequence[Gate]) -> None:
+        for g in gates:
+            self.apply_gate(state, g)
+
+    def run_part(self, state: np.ndarray, gates: Sequence[Gate], opt_slots: int = 1) -> None:
+        n = _check_state(state)
+        arr = desc_array(gates)
+        rc = self.lib.orc_run_part(_ptr(state), n, arr, len(gates), int(opt_slots))
+        if rc:
+            raise ValueError(f"oracle rejected part (code {rc})")
+
+    def permute(self, state: np.ndarray, new_slot: Sequence[int]) -> np.ndarray:
+        n = _check_state(state)
+        out = np.empty_like(state)
+        ns = (ctypes.c_int * n)(*new_slot)
+        rc = self.lib.orc_permute(_ptr(state), _ptr(out), n, ns)
+        if rc:
+            raise ValueError("bad permutation")
+        return out
+
+    def exchange_order(self, slot_to_qubit: Sequence[int], moved: Sequence[int]) -> list:
+        n = len(slot_to_qubit)
+        out = (ctypes.c_int * n)()
+        self.lib.orc_exchange_order(n, (ctypes.c_int * n)(*slot_to_qubit),
+                                    (ctypes.c_int * max(1, len(moved)))(*moved), len(moved), out)
+        return list(out)
+
+    def norm2(self, state: np.ndarray) -> float:
+        return self.lib.orc_norm2(_ptr(state), _check_state(state))
+
+    def probability(self, state: np.ndarray, qubit: int, value: int = 0) -> float:
+        return self.lib.orc_probability(_ptr(state), _check_state(state), int(qubit), int(value))
+
+    def measure_sorted(self, state: np.ndarray, qubits: Sequence[int], sorted_u: np.ndarray) -> np.ndarray:
+        n = _check_state(state)
+        u = np.ascontiguousarray(sorted_u, dtype=np.float64)
+        out = np.zeros((len(u), len(qubits)), dtype=np.int32)
+        self.lib.orc_measure_sorted(_ptr(state), n, (ctypes.c_int * max(1, len(qubits)))(*qubits), len(qubits),
+                                    _ptr(u), len(u), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        return out
+
+
+class RefSim:
+    """ctypes front end of _ref/libref.so (the reference headers)."""
+
+    PATH = os.path.join(HERE, "_ref", "libref.so")
+
+    def __init__(self, path: str | None = None):
+        path = path or self.PATH
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle ref` where the reference exists")
+        L = self.lib = ctypes.CDLL(path)
+        L.ref_create.restype = ctypes.c_void_p
+        L.ref_create.argtypes = [ctypes.c_int]
+        L.ref_destroy.argtypes = [ctypes.c_void_p]
+        L.ref_write.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.ref_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.ref_apply_gate.argtypes = [ctypes.c_void_p, ctypes.POINTER(GateDesc)]
+        L.ref_run_part.argtypes = [ctypes.c_void_p, ctypes.POINTER(GateDesc), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.ref_set_threads.argtypes = [ctypes.c_int]
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.exists(cls.PATH)
+
+    def set_threads(self, n: int):
+        self.lib.ref_set_threads(int(n))
+
+    def _with_state(self, state: np.ndarray, fn):
+        n = _check_state(state)
+        sv = self.lib.ref_create(n)
+        try:
+            self.lib.ref_write(sv, _ptr(state), n)
+            rc = fn(sv, n)
+            self.lib.ref_read(sv, _ptr(state), n)
+        finally:
+            self.lib.ref_destroy(sv)
+        if rc:
+            raise ValueError(f"reference harness refused the call (code {rc})")
+
+    def apply_gates(self, state: np.ndarray, gates: Sequence[Gate]) -> None:
+        descs = [to_desc(g) for g in gates]
+
+        def run(sv, n):
+            for d in descs:
+                rc = self.lib.ref_apply_gate(sv, ctypes.byref(d))
+                if rc:
+                    return rc
+            return 0
+        self._with_state(state, run)
+
+    def time_from_zero(self, n: int, gates: Sequence[Gate] = (), parts: Sequence[Sequence[Gate]] = (),
+                       opt_slots: int = 1) -> float:
+        """Seconds the reference takes to apply `gates` per gate, then `parts`
+        through its part driver, to a fresh |0..0> state (allocation and
+        zeroing excluded)."""
+        import time
+        descs = [to_desc(g) for g in gates]
+        arrs = [desc_array(p) for p in parts]
+        sv = self.lib.ref_create(n)
+        try:
+            t0 = time.perf_counter()
+            for d in descs:
+                if self.lib.ref_apply_gate(sv, ctypes.byref(d)):
+                    raise ValueError("reference refused a gate")
+            for p, a in zip(parts, arrs):
+                if self.lib.ref_run_part(sv, a, len(p), int(opt_slots), n):
+                    raise ValueError("reference refused a part")
+            return time.perf_counter() - t0
+        finally:
+            self.lib.ref_destroy(sv)
+
+    def run_parts(self, state: np.ndarray, parts: Sequence[Sequence[Gate]], opt_slots: int = 1) -> None:
+        arrs = [desc_array(p) for p in parts]
+
+        def run(sv, n):
+            for p, a in zip(parts, arrs):
+                rc = self.lib.ref_run_part(sv, a, len(p), int(opt_slots), n)
+                if rc:
+                    return rc
+            return 0
+        self._with_state(state, run)

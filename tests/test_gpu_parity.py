@@ -1,0 +1,260 @@
+"""GPU parity: libhsv (through its C ABI) against the CPU oracle.
+
+Bars: <= 1e-12 absolute per amplitude for the FP64 per-gate, part and fused
+FP64 paths; infidelity <= 1e-6 for the FP32-split fused path; bit-exact for
+permutations and measurement outcomes. Sizes are ones the oracle finishes in
+seconds; the closed-form tests (BV, QFT) run at larger n.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import bit_reverse_perm, rand_state
+from gatesets import (ALL_ONE, ALL_TWO, UNITARY_ONE, UNITARY_TWO, params_for, random_circuit,
+                      random_parts)
+from hisvsim_amd import capi
+from hisvsim_amd.circuit import Gate, bernstein_vazirani, qft, random_layered
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def gpu_run(n, state, fn, dtype="complex128"):
+    with capi.State(n, dtype) as s:
+        s.write(state)
+        fn(s)
+        s.synchronize()
+        return s.read()
+
+
+def per_gate_cases():
+    cases = []
+    for name in ALL_ONE:
+        for t in (0, 1, 2, 5, 9):
+            for ctl in ((), (3,) if t != 3 else (4,), (8, 6) if t not in (8, 6) else (7, 4)):
+                cases.append((name, (t,), ctl))
+    for name in ALL_TWO:
+        for t in ((0, 1), (1, 0), (2, 7), (9, 3), (4, 5)):
+            for ctl in ((), tuple(q for q in (6, 8) if q not in t)[:1], (0, 8) if 0 not in t and 8 not in t else (6,)):
+                cases.append((name, t, ctl))
+    return cases
+
+
+@pytest.mark.parametrize("name,targets,controls", per_gate_cases())
+def test_per_gate_matches_oracle(orc, name, targets, controls):
+    n = 10
+    rng = np.random.default_rng(abs(hash((name, targets, controls))) % 2 ** 32)
+    g = Gate(name, targets, controls, params_for(name, rng))
+    a = rand_state(n, 1)
+    want = a.copy()
+    orc.apply_gate(want, g)
+    got = gpu_run(n, a, lambda s: s.apply_gate(g))
+    assert np.abs(got - want).max() <= TOL
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_circuit_per_gate(orc, seed):
+    n = 12
+    rng = np.random.default_rng(seed)
+    gates = random_circuit(rng, n, 80, names=ALL_ONE + ALL_TWO, max_controls=3)
+    a = rand_state(n, seed)
+    want = a.copy()
+    orc.apply_gates(want, gates)
+    got = gpu_run(n, a, lambda s: s.apply_gates(gates))
+    assert np.abs(got - want).max() <= TOL
+
+
+@pytest.mark.parametrize("k,opt_slots,seed", [(3, 0, 0), (4, 1, 1), (6, 0, 2), (6, 1, 3), (8, 1, 4), (10, 0, 5), (11, 1, 6)])
+def test_parts_match_oracle(orc, k, opt_slots, seed):
+    n = 14
+    rng = np.random.default_rng(100 + seed)
+    parts = random_parts(rng, n, 6, k, 12, names=ALL_ONE + ALL_TWO, max_controls=2)
+    a = rand_state(n, seed)
+    want = a.copy()
+    for p in parts:
+        orc.run_part(want, p, opt_slots)
+    per_gate = a.copy()
+    for p in parts:
+        orc.apply_gates(per_gate, p)
+    assert np.array_equal(want, per_gate)  # the oracle's two paths agree bit for bit
+
+    def run(s):
+        for p in parts:
+            s.apply_part(p, opt_slots)
+    got = gpu_run(n, a, run)
+    assert np.abs(got - want).max() <= TOL
+
+
+def test_part_wider_than_lds_falls_back(orc):
+    n = 18
+    rng = np.random.default_rng(7)
+    parts = random_parts(rng, n, 2, 16, 30)
+    a = rand_state(n, 3)
+    want = a.copy()
+    for p in parts:
+        orc.apply_gates(want, p)
+    got = gpu_run(n, a, lambda s: [s.apply_part(p) for p in parts])
+    assert np.abs(got - want).max() <= TOL
+
+
+@pytest.mark.parametrize("k,seed", [(1, 0), (3, 1), (4, 2), (5, 3), (6, 4), (7, 5)])
+def test_fused_f64_matches_oracle(orc, k, seed):
+    n = 13
+    rng = np.random.default_rng(200 + seed)
+    parts = random_parts(rng, n, 4, k, 3 * k + 2, names=UNITARY_ONE + (UNITARY_TWO if k >= 2 else []),
+                         max_controls=min(2, k - 1) if k > 1 else 0)
+    a = rand_state(n, seed)
+    want = a.copy()
+    for p in parts:
+        orc.apply_gates(want, p)
+    got = gpu_run(n, a, lambda s: [s.apply_fused(p, "f64") for p in parts])
+    assert np.abs(got - want).max() <= TOL
+
+
+@pytest.mark.parametrize("k,seed", [(4, 0), (5, 1), (6, 2), (7, 3)])
+def test_fused_f32split_infidelity(orc, k, seed):
+    n = 14
+    rng = np.random.default_rng(300 + seed)
+    parts = random_parts(rng, n, 6, k, 3 * k, names=UNITARY_ONE + UNITARY_TWO, max_controls=1)
+    a = rand_state(n, seed)
+    want = a.copy()
+    for p in parts:
+        orc.apply_gates(want, p)
+    got = gpu_run(n, a, lambda s: [s.apply_fused(p, "f32split") for p in parts])
+    infid = 1 - abs(np.vdot(want, got)) ** 2
+    assert infid <= 1e-6
+    assert np.abs(got - want).max() <= 1e-6
+
+
+def test_fused_rejects_non_unitary():
+    g = Gate("C2", (0, 1), (), tuple(complex(i + 1, 0) for i in range(16)))
+    with capi.State(8) as s:
+        with pytest.raises(capi.HsvError, match="UNITARY"):
+            s.apply_fused([g])
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_permute_matches_oracle(orc, seed):
+    n = 13
+    rng = np.random.default_rng(seed)
+    perm = [int(x) for x in rng.permutation(n)]
+    a = rand_state(n, seed)
+    want = orc.permute(a, perm)
+    got = gpu_run(n, a, lambda s: s.permute_qubits(perm))
+    assert np.array_equal(got, want)
+
+
+def test_exchange_order_permutation(orc):
+    # the reference's qubit-exchange order (svsim-mpi.hpp:139-149) as a device permutation
+    n = 12
+    order = orc.exchange_order(list(range(n)), [2, 9, 5])
+    new_slot = [0] * n
+    for slot, q in enumerate(order):
+        new_slot[q] = slot
+    a = rand_state(n, 5)
+    want = orc.permute(a, new_slot)
+    got = gpu_run(n, a, lambda s: s.permute_qubits(new_slot))
+    assert np.array_equal(got, want)
+
+
+def test_reductions(orc):
+    n = 15
+    a = rand_state(n, 11) * 0.9
+    b = rand_state(n, 12)
+    with capi.State(n) as s, capi.State(n) as t:
+        s.write(a)
+        t.write(b)
+        assert abs(s.norm2() - orc.norm2(a)) <= 1e-13
+        ip = s.inner(t)
+        assert abs(ip - np.vdot(a, b)) <= 1e-13
+        assert abs(s.max_abs_diff(t) - np.abs(a - b).max()) <= 1e-15
+        for q in (0, 7, 14):
+            for v in (0, 1):
+                assert abs(s.probability(q, v) - orc.probability(a, q, v)) <= 1e-12
+
+
+def test_measure_matches_oracle(orc):
+    n = 12
+    a = rand_state(n, 21)
+    u = np.sort(np.random.default_rng(3).uniform(0, 1, 200))
+    qubits = [0, 5, 11, 3]
+    want = orc.measure_sorted(a, qubits, u)
+    with capi.State(n) as s:
+        s.write(a)
+        got = s.measure(qubits, u)
+    assert np.array_equal(got, want)
+
+
+def test_bv_closed_form():
+    n = 24
+    gates, mask = bernstein_vazirani(n, seed=5)
+    with capi.State(n) as s:
+        s.apply_gates(gates)
+        amps = s.read()
+    want = np.zeros(2 ** n, complex)
+    want[mask] = 1 / math.sqrt(2)
+    want[mask + 2 ** (n - 1)] = -1 / math.sqrt(2)
+    assert np.abs(amps - want).max() <= TOL
+
+
+def test_qft_closed_form():
+    # y[rev(k)] = N^-1/2 sum_j x[rev(j)] e^{+2 pi i jk/N}
+    n = 18
+    x = rand_state(n, 4)
+    rev = bit_reverse_perm(n)
+    want = np.empty_like(x)
+    want[rev] = np.fft.ifft(x[rev]) * math.sqrt(2 ** n)
+    got = gpu_run(n, x, lambda s: s.apply_gates(qft(n)))
+    assert np.abs(got - want).max() <= 1e-11
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_complex64_state(orc, seed):
+    n = 12
+    rng = np.random.default_rng(seed)
+    gates = random_circuit(rng, n, 40)
+    a = rand_state(n, seed)
+    want = a.copy()
+    orc.apply_gates(want, gates)
+    got = gpu_run(n, a.astype(np.complex64).astype(np.complex128), lambda s: s.apply_gates(gates), "complex64")
+    assert np.abs(got - want).max() <= 1e-5
+    parts = random_parts(rng, n, 4, 5, 8)
+    want = a.copy()
+    for p in parts:
+        orc.apply_gates(want, p)
+    got = gpu_run(n, a, lambda s: [s.apply_part(p) for p in parts], "complex64")
+    assert np.abs(got - want).max() <= 1e-5
+    got = gpu_run(n, a, lambda s: [s.apply_fused(p, "f32split") for p in parts], "complex64")
+    assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
+
+
+def test_modes_agree_at_26_qubits():
+    """Size-independent properties at a larger n: the per-gate, LDS-part and
+    fused-FP64 paths agree to 1e-12 and preserve the norm."""
+    n = 26
+    gates = random_layered(n, 2, seed=3)
+    from hisvsim_amd.partition import layered_parts
+    parts = layered_parts(gates, n, 6)
+    with capi.State(n) as a, capi.State(n) as b, capi.State(n) as c:
+        a.apply_gates(gates)
+        for p in parts:
+            b.apply_part(p)
+        for p in parts:
+            c.apply_fused(p, "f64")
+        assert a.max_abs_diff(b) <= TOL
+        assert a.max_abs_diff(c) <= TOL
+        assert abs(a.norm2() - 1) <= 1e-12
+
+
+def test_errors_are_reported():
+    with capi.State(6) as s:
+        with pytest.raises(capi.HsvError, match="ARG"):
+            s.apply_gate(Gate("H", (6,)))
+        with pytest.raises(capi.HsvError, match="ARG"):
+            s.apply_gate(Gate("X", (1,), (1,)))
+        with pytest.raises(capi.HsvError, match="GATE"):
+            s.apply_gate(Gate("SWAP", (1,)))
+        with pytest.raises(capi.HsvError, match="ARG"):
+            s.permute_qubits([0, 0, 1, 2, 3, 4])
