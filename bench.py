@@ -41,6 +41,18 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+LOW_SLOTS = 5  # qubits the device adds to every LDS-staged part (libhsv default)
+
+
+def make_parts(gates, mode: str, k: int):
+    """Greedy acyclic parts. For the LDS/register part kernel, k counts the
+    low qubits the device adds to every part anyway, so the partitioner
+    starts each part with them."""
+    if mode == "part":
+        return greedy_parts(gates, k, always=range(LOW_SLOTS))
+    return greedy_parts(gates, k)
+
+
 def run_path(s: capi.State, mode: str, gates, parts) -> None:
     s.reset()
     if mode == "pergate":
@@ -58,7 +70,7 @@ def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0) -> d
     """Infidelity and max |d amp| of the chosen path against the FP64 oracle."""
     from oracle.oracle import Oracle
     gates = random_layered(n, depth, seed=seed)
-    parts = greedy_parts(gates, k)
+    parts = make_parts(gates, mode, k)
     want = np.zeros(2 ** n, complex)
     want[0] = 1
     o = Oracle()
@@ -119,10 +131,10 @@ def main() -> int:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU layout is described "
               f"in DESIGN.md and not run here.", file=sys.stderr)
         return 2
-    k = args.k or {"part": 10, "fused64": 5, "fused32": 5, "pergate": 1}[args.mode]
+    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "pergate": 1}[args.mode]
     n, depth = args.n, args.depth
     gates = random_layered(n, depth, seed=0)
-    parts = greedy_parts(gates, k) if args.mode != "pergate" else []
+    parts = make_parts(gates, args.mode, k) if args.mode != "pergate" else []
     S = 16
     total_upd = sum(gate_amplitudes(g, n) for g in gates)
     log(f"[bench] n={n} depth={depth} gates={len(gates)} mode={args.mode} k={k} parts={len(parts)}")

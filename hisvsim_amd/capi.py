@@ -99,6 +99,7 @@ SIGNATURES = {
     "hsv_gate_matrix": (ctypes.c_int, [_G, _D, _I]),
     "hsv_fused_unitary": (ctypes.c_int, [ctypes.c_int, _G, ctypes.c_int, _I, _I, _D, _D]),
     "hsv_synchronize": (ctypes.c_int, [_P]),
+    "hsv_tune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
     "hsv_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "hsv_kernel_times": (ctypes.c_int, [_P, _D, ctypes.c_int, _I]),
     "hsv_set_opt_slots": (None, [ctypes.c_int]),
@@ -252,6 +253,9 @@ class State:
         _check(lib().hsv_measure(self._h, q, len(qubits), u.ctypes.data_as(_D), len(u),
                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return out
+
+    def tune(self, key: str, value: int) -> None:
+        _check(lib().hsv_tune(self._h, key.encode(), int(value)))
 
     # -- sync / timing ----------------------------------------------------------
     def synchronize(self) -> None:

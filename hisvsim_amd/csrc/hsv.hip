@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -56,6 +57,14 @@ struct hsv_state {
   int timing = 0;
   int max_lds = 65536;
   int num_cu = 256;
+  // tuning knobs (hsv_tune)
+  int reg_waves = 2;                 // persistent-grid factor of the register part kernel
+  int part_kernel = 1;               // 1: register-blocked part kernel for k >= 8; 0: LDS kernel only
+  int reg_bits = 0;                  // preferred register bits (0: per dtype)
+  // Slots 0..low_slots-1 join every LDS-staged part when they fit: gathered
+  // runs of 2^low_slots amplitudes (512 B for complex128 at 5) sustain
+  // 5.0 TB/s on MI355X against 3.9 TB/s for 128 B runs.
+  int low_slots = 5;
   DeviceScratch fused_scratch;       // fused unitary + offset tables
   size_t elem() const { return dtype == HSV_C128 ? 16 : 8; }
 };
@@ -124,27 +133,40 @@ int grid_for(uint64_t items, uint64_t per_block, uint64_t cap) {
   return (int)std::min<uint64_t>(g, cap);
 }
 
+template <typename R, int ITEMS, bool NT>
+int launch_gate_k(hsv_state* s, const Lowered& L, const GateArgs& g) {
+  const int grid = grid_for(g.nwork, uint64_t(kGateBlock) * ITEMS, 1u << 20);
+  auto* a = amp_as<R>(s);
+  switch (L.kind) {
+    case K_PHASE1: hipLaunchKernelGGL((k_gate<R, GK_PHASE1, ITEMS, NT>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_DIAG1: hipLaunchKernelGGL((k_gate<R, GK_DIAG1, ITEMS, NT>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_XSWAP1: hipLaunchKernelGGL((k_gate<R, GK_XSWAP1, ITEMS, NT>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_DENSE1: hipLaunchKernelGGL((k_gate<R, GK_DENSE1, ITEMS, NT>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_DIAG2: hipLaunchKernelGGL((k_gate<R, GK_DIAG2, ITEMS, NT>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_SWAP2: hipLaunchKernelGGL((k_gate<R, GK_SWAP2, ITEMS, NT>), grid, kGateBlock, 0, s->stream, a, g); break;
+    case K_DENSE2: {
+      constexpr int IT2 = ITEMS > 4 ? 4 : ITEMS;
+      hipLaunchKernelGGL((k_gate<R, GK_DENSE2, IT2, NT>), grid_for(g.nwork, uint64_t(kGateBlock) * IT2, 1u << 20),
+                         kGateBlock, 0, s->stream, a, g);
+      break;
+    }
+    default: return HSV_OK;
+  }
+  HIPCHK(hipGetLastError());
+  return HSV_OK;
+}
+
 template <typename R>
 int launch_gate_t(hsv_state* s, const Lowered& L) {
   GateArgs g{};
   sweep_geometry(L, s->n, [](int q) { return q; }, &g.nfixed, g.fixed, &g.setmask, &g.off1, &g.off2, &g.nwork);
   pack_matrix(L, g.m);
-  constexpr int ITEMS = 4;
-  const int grid = grid_for(g.nwork, uint64_t(kGateBlock) * ITEMS, 1u << 20);
-  auto* a = amp_as<R>(s);
-  switch (L.kind) {
-    case K_PHASE1: hipLaunchKernelGGL((k_gate<R, GK_PHASE1, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
-    case K_DIAG1: hipLaunchKernelGGL((k_gate<R, GK_DIAG1, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
-    case K_XSWAP1: hipLaunchKernelGGL((k_gate<R, GK_XSWAP1, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
-    case K_DENSE1: hipLaunchKernelGGL((k_gate<R, GK_DENSE1, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
-    case K_DIAG2: hipLaunchKernelGGL((k_gate<R, GK_DIAG2, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
-    case K_SWAP2: hipLaunchKernelGGL((k_gate<R, GK_SWAP2, ITEMS>), grid, kGateBlock, 0, s->stream, a, g); break;
-    case K_DENSE2: hipLaunchKernelGGL((k_gate<R, GK_DENSE2, 2>), grid_for(g.nwork, uint64_t(kGateBlock) * 2, 1u << 20),
-                                      kGateBlock, 0, s->stream, a, g); break;
-    default: return HSV_OK;
-  }
-  HIPCHK(hipGetLastError());
-  return HSV_OK;
+  // Measured on MI355X (2^31 amplitudes): when the pair stride is >= 8
+  // amplitudes, 2 items per thread with non-temporal access sustain the most
+  // bandwidth; for the lowest targets, 8 items per thread with cached access.
+  const uint64_t min_off = L.nt == 2 ? std::min(g.off1, g.off2) : g.off1;
+  if (min_off >= 8) return launch_gate_k<R, 2, true>(s, L, g);
+  return launch_gate_k<R, 8, false>(s, L, g);
 }
 
 int launch_gate(hsv_state* s, const Lowered& L) {
@@ -212,6 +234,186 @@ int launch_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t part_ma
   }
   HIPCHK(hipGetLastError());
   return HSV_OK;
+}
+
+// ---- register-blocked part plan ---------------------------------------------
+bool is_diagonal(int kind) { return kind == K_PHASE1 || kind == K_DIAG1 || kind == K_DIAG2; }
+
+struct RegPlan {
+  std::vector<RegSeg> segs;
+  std::vector<RegGate> gates;
+  std::vector<int> order;   // index into the part's gate list, in plan order
+  std::vector<double> mat2; // 32 doubles per dense two-qubit gate
+  std::vector<int> m2idx;   // per gate: its matrix in mat2, or -1
+};
+
+// Picks the next layout: the targets of the earliest pending non-diagonal
+// gates, topped up with bits of the current layout, then with high bits
+// (low inner bits stay thread bits when possible, for coalesced access).
+uint32_t choose_layout(const std::vector<int>& pending, const std::vector<uint32_t>& need_bits,
+                       const std::vector<std::array<int, 2>>& need_list, uint32_t cur, int k, int RB) {
+  uint32_t want = 0;
+  int cnt = 0;
+  for (int i : pending) {
+    if (!need_bits[i]) continue;
+    if (cnt + __builtin_popcount(need_bits[i] & ~want) > RB) break;
+    for (int t : need_list[i])
+      if (t >= 0 && !(want >> t & 1)) { want |= 1u << t; ++cnt; }
+    if (cnt == RB) break;
+  }
+  for (int b = k - 1; b >= 0 && cnt < RB; --b)
+    if ((cur >> b & 1) && !(want >> b & 1)) { want |= 1u << b; ++cnt; }
+  for (int b = k - 1; b >= 0 && cnt < RB; --b)
+    if (!(want >> b & 1)) { want |= 1u << b; ++cnt; }
+  return want;
+}
+
+// Splits a part (gates in inner-bit coordinates) into layout segments. Gates
+// may move ahead of later gates only past gates on disjoint qubits, so the
+// result applies the same operator as the part in its given order.
+void plan_register_part(const std::vector<Lowered>& Ls, const int* inner_of, int k, int RB, RegPlan* P) {
+  const int ng = (int)Ls.size();
+  std::vector<uint32_t> qmask(ng, 0), need_bits(ng, 0);
+  std::vector<std::array<int, 2>> need_list(ng, {-1, -1});
+  std::vector<int> pending;
+  for (int i = 0; i < ng; ++i) {
+    const Lowered& L = Ls[i];
+    if (L.kind == K_IDENT) continue;
+    for (int t = 0; t < L.nt; ++t) qmask[i] |= 1u << inner_of[L.t[t]];
+    for (int c = 0; c < L.nc; ++c) qmask[i] |= 1u << inner_of[L.c[c]];
+    if (!is_diagonal(L.kind))
+      for (int t = 0; t < L.nt; ++t) {
+        need_bits[i] |= 1u << inner_of[L.t[t]];
+        need_list[i][t] = inner_of[L.t[t]];
+      }
+    pending.push_back(i);
+  }
+  uint32_t cur = choose_layout(pending, need_bits, need_list, 0, k, RB);
+  P->segs.clear();
+  P->gates.clear();
+  P->order.clear();
+  P->mat2.clear();
+  P->m2idx.clear();
+  do {
+    uint32_t blocked = 0;
+    std::vector<int> applied, rest;
+    for (int i : pending) {
+      if (qmask[i] & blocked) { blocked |= qmask[i]; rest.push_back(i); continue; }
+      if ((need_bits[i] & ~cur) == 0) applied.push_back(i);
+      else { blocked |= qmask[i]; rest.push_back(i); }
+    }
+    RegSeg seg{cur, (int)P->gates.size(), (int)applied.size(), 0};
+    P->segs.push_back(seg);
+    for (int i : applied) {
+      const Lowered& L = Ls[i];
+      RegGate G{};
+      G.kind = L.kind;
+      G.t0bit = 1u << inner_of[L.t[0]];
+      G.t1bit = L.nt == 2 ? 1u << inner_of[L.t[1]] : 0;
+      G.p0 = __builtin_popcount(cur & (G.t0bit - 1));
+      G.p1 = L.nt == 2 ? __builtin_popcount(cur & (G.t1bit - 1)) : 0;
+      for (int c = 0; c < L.nc; ++c) G.cmask |= 1u << inner_of[L.c[c]];
+      if (L.kind == K_PHASE1) G.cmask |= G.t0bit;
+      double m[32];
+      pack_matrix(L, m);
+      std::memcpy(G.m, m, sizeof(G.m));
+      int m2 = -1;
+      if (L.kind == K_DENSE2) {
+        m2 = (int)(P->mat2.size() / 32);
+        P->mat2.insert(P->mat2.end(), m, m + 32);
+      }
+      P->gates.push_back(G);
+      P->order.push_back(i);
+      P->m2idx.push_back(m2);
+    }
+    pending.swap(rest);
+    if (!pending.empty()) cur = choose_layout(pending, need_bits, need_list, cur, k, RB);
+  } while (!pending.empty());
+}
+
+// Register bits and thread bits for a k-qubit part (k >= 8). With complex128,
+// 16 values per thread (RB = 4) fit in 159 VGPRs (3 waves/SIMD) while 32 take
+// all 512 registers; complex64 affords RB = 5 at 174 VGPRs.
+// Instantiated (TB, RB) shapes; the nearest one to the preferred RB is used.
+// With the prefetch buffer a complex128 thread holds 2 x 16 values in 252
+// VGPRs; 2 x 32 would spill, so complex128 stops at RB = 4.
+bool reg_shape_available(int tb, int rb, int elem) {
+  if (elem == 16 && rb > 4) return false;
+  switch (tb) {
+    case 6: return rb >= 2 && rb <= 5;
+    case 7: case 8: return rb == 4 || rb == 5;
+    case 9: return rb == 4;
+    default: return false;
+  }
+}
+bool reg_shape(int k, int rb_pref, int elem, int* TB, int* RB) {
+  for (int d = 0; d <= 3; ++d)
+    for (int rb : {rb_pref + d, rb_pref - d})
+      if (reg_shape_available(k - rb, rb, elem)) {
+        *TB = k - rb;
+        *RB = rb;
+        return true;
+      }
+  return false;
+}
+
+template <typename R, int TB, int RB>
+int launch_reg(hsv_state* s, const RegPlan& P, uint64_t nbases, uint64_t outer, uint64_t part_mask) {
+  // device layout: segments | gate records | 4x4 matrices
+  const size_t seg_bytes = P.segs.size() * sizeof(RegSeg);
+  const size_t gate_bytes = P.gates.size() * sizeof(RegGate);
+  const size_t bytes = seg_bytes + gate_bytes + P.mat2.size() * sizeof(double);
+  if (int rc = ensure_gate_buf(s, bytes)) return rc;
+  std::vector<unsigned char> host(bytes);
+  std::memcpy(host.data(), P.segs.data(), seg_bytes);
+  const double* dev_mat2 =
+      reinterpret_cast<const double*>(static_cast<const unsigned char*>(s->gate_buf) + seg_bytes + gate_bytes);
+  RegGate* hg = reinterpret_cast<RegGate*>(host.data() + seg_bytes);
+  for (size_t i = 0; i < P.gates.size(); ++i) {
+    hg[i] = P.gates[i];
+    hg[i].m2 = P.m2idx[i] >= 0 ? dev_mat2 + 32 * size_t(P.m2idx[i]) : nullptr;
+  }
+  if (!P.mat2.empty()) std::memcpy(host.data() + seg_bytes + gate_bytes, P.mat2.data(), P.mat2.size() * sizeof(double));
+  HIPCHK(hipMemcpyAsync(s->gate_buf, host.data(), bytes, hipMemcpyHostToDevice, s->stream));
+  const auto* segs = static_cast<const RegSeg*>(s->gate_buf);
+  const auto* gates = reinterpret_cast<const RegGate*>(static_cast<const unsigned char*>(s->gate_buf) + seg_bytes);
+  constexpr int K = TB + RB;
+  const size_t lds = P.segs.size() > 1 ? (size_t(1) << K) * sizeof(typename VecT<R>::type) : 0;
+  auto kern = &k_part_reg<R, TB, RB, true>;
+  if (lds > 65536)
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+  const int per_cu = lds > 65536 ? 1 : lds > 32768 ? 2 : 4;
+  const int grid = (int)std::min<uint64_t>(nbases, uint64_t(s->num_cu) * per_cu * s->reg_waves);
+  hipLaunchKernelGGL(kern, grid, 1 << TB, lds, s->stream, amp_as<R>(s), nbases, outer, part_mask, segs,
+                     (int)P.segs.size(), gates);
+  HIPCHK(hipGetLastError());
+  return HSV_OK;
+}
+
+template <typename R>
+int launch_reg_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t part_mask) {
+  const int k = __builtin_popcountll(part_mask);
+  int TB, RB;
+  const int rb_pref = s->reg_bits > 0 ? s->reg_bits : (sizeof(R) == 8 ? 4 : 5);
+  if (!reg_shape(k, rb_pref, (int)sizeof(typename VecT<R>::type), &TB, &RB))
+    return fail(HSV_ERR_LIMIT, "no register-kernel shape for a " + std::to_string(k) + "-qubit part");
+  int inner_of[64];
+  for (int b = 0, r = 0; b < 64; ++b) inner_of[b] = (part_mask >> b & 1) ? r++ : -1;
+  RegPlan P;
+  plan_register_part(Ls, inner_of, k, RB, &P);
+  const uint64_t all = s->n == 64 ? ~0ull : ((1ull << s->n) - 1);
+  const uint64_t outer = all & ~part_mask;
+  const uint64_t nbases = 1ull << (s->n - k);
+#define HSV_REG_CASE(T_, R_) \
+  if (TB == T_ && RB == R_) return launch_reg<R, T_, R_>(s, P, nbases, outer, part_mask);
+  HSV_REG_CASE(6, 2) HSV_REG_CASE(6, 3) HSV_REG_CASE(6, 4)
+  HSV_REG_CASE(7, 4) HSV_REG_CASE(8, 4) HSV_REG_CASE(9, 4)
+  if constexpr (sizeof(R) == 4) {
+    HSV_REG_CASE(6, 5) HSV_REG_CASE(7, 5) HSV_REG_CASE(8, 5)
+  }
+#undef HSV_REG_CASE
+  return fail(HSV_ERR_LIMIT, "no register-kernel shape for a " + std::to_string(k) + "-qubit part");
 }
 
 template <typename R, int MODE>
@@ -345,7 +547,7 @@ int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_s
   const int kmax = max_part_qubits(s);
   // Slots 0..2 make every gathered run at least 8 consecutive amplitudes; they
   // join the part whenever the LDS budget allows (results are unchanged).
-  for (int b = 0; b < 3 && b < s->n; ++b)
+  for (int b = 0; b < s->low_slots && b < s->n; ++b)
     if (!(part >> b & 1) && __builtin_popcountll(part) < kmax) part |= 1ull << b;
   if (int rc = begin_timed(s)) return rc;
   if (__builtin_popcountll(part) > kmax) {
@@ -353,7 +555,9 @@ int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_s
     for (const Lowered& L : Ls)
       if (int rc = launch_gate(s, L)) return rc;
   } else if (num_gates > 0) {
-    int rc = s->dtype == HSV_C128 ? launch_part_t<double>(s, Ls, part) : launch_part_t<float>(s, Ls, part);
+    const bool reg = s->part_kernel == 1 && __builtin_popcountll(part) >= 8;
+    int rc = reg ? (s->dtype == HSV_C128 ? launch_reg_part_t<double>(s, Ls, part) : launch_reg_part_t<float>(s, Ls, part))
+                 : (s->dtype == HSV_C128 ? launch_part_t<double>(s, Ls, part) : launch_part_t<float>(s, Ls, part));
     if (rc) return rc;
   }
   g_part_count++;
@@ -554,6 +758,17 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
 int hsv_synchronize(hsv_state* s) {
   if (!s) return fail(HSV_ERR_ARG, "null handle");
   HIPCHK(hipStreamSynchronize(s->stream));
+  return HSV_OK;
+}
+
+int hsv_tune(hsv_state* s, const char* key, int value) {
+  if (!s || !key) return fail(HSV_ERR_ARG, "null argument");
+  const std::string k = key;
+  if (k == "reg_waves" && value >= 1 && value <= 64) s->reg_waves = value;
+  else if (k == "part_kernel" && (value == 0 || value == 1)) s->part_kernel = value;
+  else if (k == "reg_bits" && value >= 0 && value <= 5) s->reg_bits = value;
+  else if (k == "low_slots" && value >= 0 && value <= 6) s->low_slots = value;
+  else return fail(HSV_ERR_ARG, "unknown tuning key or value out of range: " + k);
   return HSV_OK;
 }
 

@@ -24,11 +24,36 @@ template <typename V, typename R>
 __device__ __forceinline__ V cmul(V a, R bx, R by) {
   return cmk<V, R>(a.x * bx - a.y * by, a.x * by + a.y * bx);
 }
+// acc + a*b as two FMA chains (4 FMAs; `acc += a*b - c*d` written naively
+// compiles to mul + fma + add per component).
 template <typename V, typename R>
 __device__ __forceinline__ V cmac(V acc, V a, R bx, R by) {
-  acc.x += a.x * bx - a.y * by;
-  acc.y += a.x * by + a.y * bx;
+  acc.x = fma(a.x, bx, fma(-a.y, by, acc.x));  // HIP's fma overloads float and double
+  acc.y = fma(a.x, by, fma(a.y, bx, acc.y));
   return acc;
+}
+
+// Streaming loads/stores; NT = non-temporal (the state is touched once per
+// pass, so keeping it out of the caches leaves them to the other stream).
+template <bool NT, typename V>
+__device__ __forceinline__ V ldg(const V* p) {
+  if constexpr (NT) {
+    V v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    return v;
+  } else {
+    return *p;
+  }
+}
+template <bool NT, typename V>
+__device__ __forceinline__ void stg(V* p, V v) {
+  if constexpr (NT) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+  } else {
+    *p = v;
+  }
 }
 
 // Insert a zero bit at each position in `fixed` (ascending), then set `set`.
@@ -70,7 +95,7 @@ enum { GK_PHASE1 = 1, GK_DIAG1 = 2, GK_XSWAP1 = 3, GK_DENSE1 = 4, GK_DIAG2 = 5, 
 
 constexpr int kGateBlock = 256;
 
-template <typename R, int KIND, int ITEMS>
+template <typename R, int KIND, int ITEMS, bool NT>
 __global__ __launch_bounds__(kGateBlock) void k_gate(typename VecT<R>::type* __restrict__ a, GateArgs g) {
   using V = typename VecT<R>::type;
   constexpr uint64_t per_block = uint64_t(kGateBlock) * ITEMS;
@@ -90,13 +115,13 @@ __global__ __launch_bounds__(kGateBlock) void k_gate(typename VecT<R>::type* __r
     if constexpr (KIND == GK_PHASE1) {
       V v[ITEMS];
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) if (ok[j]) v[j] = a[idx[j]];
+      for (int j = 0; j < ITEMS; ++j) if (ok[j]) v[j] = ldg<NT>(&a[idx[j]]);
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) if (ok[j]) a[idx[j]] = cmul<V, R>(v[j], m[0], m[1]);
+      for (int j = 0; j < ITEMS; ++j) if (ok[j]) stg<NT>(&a[idx[j]], cmul<V, R>(v[j], m[0], m[1]));
     } else if constexpr (KIND == GK_DIAG1 || KIND == GK_XSWAP1 || KIND == GK_DENSE1) {
       V v0[ITEMS], v1[ITEMS];
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) if (ok[j]) { v0[j] = a[idx[j]]; v1[j] = a[idx[j] + g.off1]; }
+      for (int j = 0; j < ITEMS; ++j) if (ok[j]) { v0[j] = ldg<NT>(&a[idx[j]]); v1[j] = ldg<NT>(&a[idx[j] + g.off1]); }
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
         if (!ok[j]) continue;
@@ -111,24 +136,24 @@ __global__ __launch_bounds__(kGateBlock) void k_gate(typename VecT<R>::type* __r
           o0 = cmac<V, R>(cmul<V, R>(v0[j], m[0], m[1]), v1[j], m[2], m[3]);
           o1 = cmac<V, R>(cmul<V, R>(v0[j], m[4], m[5]), v1[j], m[6], m[7]);
         }
-        a[idx[j]] = o0;
-        a[idx[j] + g.off1] = o1;
+        stg<NT>(&a[idx[j]], o0);
+        stg<NT>(&a[idx[j] + g.off1], o1);
       }
     } else if constexpr (KIND == GK_SWAP2) {
       V v0[ITEMS], v1[ITEMS];
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) if (ok[j]) { v0[j] = a[idx[j] + g.off1]; v1[j] = a[idx[j] + g.off2]; }
+      for (int j = 0; j < ITEMS; ++j) if (ok[j]) { v0[j] = ldg<NT>(&a[idx[j] + g.off1]); v1[j] = ldg<NT>(&a[idx[j] + g.off2]); }
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) if (ok[j]) { a[idx[j] + g.off1] = v1[j]; a[idx[j] + g.off2] = v0[j]; }
+      for (int j = 0; j < ITEMS; ++j) if (ok[j]) { stg<NT>(&a[idx[j] + g.off1], v1[j]); stg<NT>(&a[idx[j] + g.off2], v0[j]); }
     } else {  // GK_DIAG2, GK_DENSE2: v[2*b(t0) + b(t1)]
       V v[ITEMS][4];
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
         if (!ok[j]) continue;
-        v[j][0] = a[idx[j]];
-        v[j][1] = a[idx[j] + g.off2];
-        v[j][2] = a[idx[j] + g.off1];
-        v[j][3] = a[idx[j] + g.off1 + g.off2];
+        v[j][0] = ldg<NT>(&a[idx[j]]);
+        v[j][1] = ldg<NT>(&a[idx[j] + g.off2]);
+        v[j][2] = ldg<NT>(&a[idx[j] + g.off1]);
+        v[j][3] = ldg<NT>(&a[idx[j] + g.off1 + g.off2]);
       }
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
@@ -146,10 +171,10 @@ __global__ __launch_bounds__(kGateBlock) void k_gate(typename VecT<R>::type* __r
             o[r] = cmac<V, R>(acc, v[j][3], m[8 * r + 6], m[8 * r + 7]);
           }
         }
-        a[idx[j]] = o[0];
-        a[idx[j] + g.off2] = o[1];
-        a[idx[j] + g.off1] = o[2];
-        a[idx[j] + g.off1 + g.off2] = o[3];
+        stg<NT>(&a[idx[j]], o[0]);
+        stg<NT>(&a[idx[j] + g.off2], o[1]);
+        stg<NT>(&a[idx[j] + g.off1], o[2]);
+        stg<NT>(&a[idx[j] + g.off1 + g.off2], o[3]);
       }
     }
   }
@@ -359,6 +384,301 @@ __global__ __launch_bounds__(256) void k_permute(const typename VecT<R>::type* _
     const uint64_t j = tab[0][i & mask] | tab[1][(i >> kPermChunk) & mask] |
                        tab[2][(i >> (2 * kPermChunk)) & mask] | tab[3][(i >> (3 * kPermChunk)) & mask];
     out[j] = in[i];
+  }
+}
+
+}  // namespace hsv
+
+// ---- register-blocked part kernel ---------------------------------------------
+// Each thread keeps NR = 2^RB of the part's 2^k amplitudes in registers; the
+// k inner bits are split into RB "register bits" (which of a thread's values)
+// and TB = k - RB "thread bits" (which thread). A layout is the set of inner
+// bits that are register bits (`regmask`); register slot r holds the inner
+// index jt | jr[r], jt = pdep(thread, ~regmask), jr[r] = pdep(r, regmask).
+// A non-diagonal gate needs its targets among the register bits and then
+// runs entirely in registers; diagonal gates and controls work in any
+// layout. The host plan splits the part into segments, each a layout plus
+// the gates it can apply; between segments the values go through LDS once to
+// change layout. Global loads and stores use the first and last layouts.
+namespace hsv {
+
+struct RegSeg {
+  uint32_t regmask;
+  int first;   // first gate of the segment
+  int count;
+  int pad;
+};
+
+// 96 bytes, so the next gate's record can be prefetched into scalar
+// registers while the current one runs; dense two-qubit matrices (32
+// doubles) live in a side array indexed by m2.
+struct RegGate {
+  int kind;
+  int p0, p1;       // register slots of targets[0] / targets[1] (non-diagonal kinds)
+  uint32_t t0bit;   // inner bit of targets[0] (diagonal kinds)
+  uint32_t t1bit;   // inner bit of targets[1]
+  uint32_t cmask;   // inner bits that must be 1 (controls; the target of a phase gate)
+  const double* m2;  // GK_DENSE2: device address of its 4x4 matrix (32 doubles)
+  double m[8];       // 2x2 matrix or up to 4 diagonal entries, (re, im)
+};
+
+// Component-wise select and copy: whole-struct `c ? a : b` and struct copies
+// make clang select between addresses and memcpy, which keeps the register
+// array in scratch memory; per-component forms keep it in VGPRs.
+template <typename V>
+__device__ __forceinline__ void vsel(V& dst, bool on, const V& a, const V& b) {
+  dst.x = on ? a.x : b.x;
+  dst.y = on ? a.y : b.y;
+}
+template <typename V, typename W>
+__device__ __forceinline__ void vcopy(V& dst, const W& src) {
+  dst.x = src.x;
+  dst.y = src.y;
+}
+
+// CTRL = false: the gate has no controls, so no per-value select is needed.
+template <typename V, typename R, int NR, int P, bool CTRL>
+__device__ __forceinline__ void reg_dense1_impl(V* v, const RegGate& G, uint32_t jt, const uint32_t* jr,
+                                                bool swap_only) {
+  const R m0 = (R)G.m[0], m1 = (R)G.m[1], m2 = (R)G.m[2], m3 = (R)G.m[3];
+  const R m4 = (R)G.m[4], m5 = (R)G.m[5], m6 = (R)G.m[6], m7 = (R)G.m[7];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r & (1 << P)) continue;
+    constexpr int S = 1 << P;
+    const bool on = CTRL ? ((jt | jr[r]) & G.cmask) == G.cmask : true;
+    const V a0 = v[r], a1 = v[r + S];
+    V o0, o1;
+    if (swap_only) {
+      o0 = a1;
+      o1 = a0;
+    } else {
+      o0 = cmac<V, R>(cmul<V, R>(a0, m0, m1), a1, m2, m3);
+      o1 = cmac<V, R>(cmul<V, R>(a0, m4, m5), a1, m6, m7);
+    }
+    vsel(v[r], on, o0, a0);
+    vsel(v[r + S], on, o1, a1);
+  }
+}
+
+template <typename V, typename R, int NR, int P>
+__device__ __forceinline__ void reg_dense1(V* v, const RegGate& G, uint32_t jt, const uint32_t* jr, bool swap_only) {
+  if (G.cmask) reg_dense1_impl<V, R, NR, P, true>(v, G, jt, jr, swap_only);
+  else reg_dense1_impl<V, R, NR, P, false>(v, G, jt, jr, swap_only);
+}
+
+template <typename V, typename R, int NR, int P0, int P1>
+__device__ __forceinline__ void reg_dense2(V* v, const RegGate& G, uint32_t jt, const uint32_t* jr, bool swap_only) {
+  constexpr int S0 = 1 << P0, S1 = 1 << P1;
+  const bool ctrl = G.cmask != 0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if ((r & S0) || (r & S1)) continue;
+    const bool on = !ctrl || ((jt | jr[r]) & G.cmask) == G.cmask;
+    const V x0 = v[r], x1 = v[r + S1], x2 = v[r + S0], x3 = v[r + S0 + S1];
+    V o[4];
+    if (swap_only) {
+      o[0] = x0; o[1] = x2; o[2] = x1; o[3] = x3;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double* M = G.m2;
+        V acc = cmul<V, R>(x0, (R)M[8 * q], (R)M[8 * q + 1]);
+        acc = cmac<V, R>(acc, x1, (R)M[8 * q + 2], (R)M[8 * q + 3]);
+        acc = cmac<V, R>(acc, x2, (R)M[8 * q + 4], (R)M[8 * q + 5]);
+        o[q] = cmac<V, R>(acc, x3, (R)M[8 * q + 6], (R)M[8 * q + 7]);
+      }
+    }
+    vsel(v[r], on, o[0], x0);
+    vsel(v[r + S1], on, o[1], x1);
+    vsel(v[r + S0], on, o[2], x2);
+    vsel(v[r + S0 + S1], on, o[3], x3);
+  }
+}
+
+template <typename V, typename R, int NR, int RB, int P0>
+__device__ __forceinline__ void reg_dense2_p1(V* v, const RegGate& G, uint32_t jt, const uint32_t* jr, bool sw) {
+  switch (G.p1) {
+    case 0: if constexpr (P0 != 0) reg_dense2<V, R, NR, P0, 0>(v, G, jt, jr, sw); break;
+    case 1: if constexpr (P0 != 1 && RB > 1) reg_dense2<V, R, NR, P0, 1>(v, G, jt, jr, sw); break;
+    case 2: if constexpr (P0 != 2 && RB > 2) reg_dense2<V, R, NR, P0, 2>(v, G, jt, jr, sw); break;
+    case 3: if constexpr (P0 != 3 && RB > 3) reg_dense2<V, R, NR, P0, 3>(v, G, jt, jr, sw); break;
+    case 4: if constexpr (P0 != 4 && RB > 4) reg_dense2<V, R, NR, P0, 4>(v, G, jt, jr, sw); break;
+    case 5: if constexpr (P0 != 5 && RB > 5) reg_dense2<V, R, NR, P0, 5>(v, G, jt, jr, sw); break;
+    default: break;
+  }
+}
+
+template <typename V, typename R, int NR, int RB>
+__device__ __forceinline__ void reg_apply(V* v, const RegGate& G, uint32_t jt, const uint32_t* jr) {
+  const int kind = G.kind;
+  if (kind == GK_PHASE1 && G.m[0] == -1.0 && G.m[1] == 0.0) {
+    // Z / CZ / multi-controlled Z: a sign flip, no multiply
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const bool on = ((jt | jr[r]) & G.cmask) == G.cmask;
+      v[r].x = on ? -v[r].x : v[r].x;
+      v[r].y = on ? -v[r].y : v[r].y;
+    }
+  } else if (kind == GK_PHASE1 || kind == GK_DIAG1 || kind == GK_DIAG2) {
+    const R d0x = (R)G.m[0], d0y = (R)G.m[1], d1x = (R)G.m[2], d1y = (R)G.m[3];
+    const R d2x = (R)G.m[4], d2y = (R)G.m[5], d3x = (R)G.m[6], d3y = (R)G.m[7];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const uint32_t j = jt | jr[r];
+      const bool on = (j & G.cmask) == G.cmask;
+      R dx, dy;
+      if (kind == GK_PHASE1) {
+        dx = d0x; dy = d0y;
+      } else if (kind == GK_DIAG1) {
+        const bool b = j & G.t0bit;
+        dx = b ? d1x : d0x; dy = b ? d1y : d0y;
+      } else {
+        const bool b0 = j & G.t0bit, b1 = j & G.t1bit;
+        dx = b0 ? (b1 ? d3x : d2x) : (b1 ? d1x : d0x);
+        dy = b0 ? (b1 ? d3y : d2y) : (b1 ? d1y : d0y);
+      }
+      const V o = cmul<V, R>(v[r], dx, dy);
+      vsel(v[r], on, o, v[r]);
+    }
+  } else if (kind == GK_DENSE1 || kind == GK_XSWAP1) {
+    const bool sw = kind == GK_XSWAP1;
+    switch (G.p0) {
+      case 0: reg_dense1<V, R, NR, 0>(v, G, jt, jr, sw); break;
+      case 1: if constexpr (RB > 1) reg_dense1<V, R, NR, 1>(v, G, jt, jr, sw); break;
+      case 2: if constexpr (RB > 2) reg_dense1<V, R, NR, 2>(v, G, jt, jr, sw); break;
+      case 3: if constexpr (RB > 3) reg_dense1<V, R, NR, 3>(v, G, jt, jr, sw); break;
+      case 4: if constexpr (RB > 4) reg_dense1<V, R, NR, 4>(v, G, jt, jr, sw); break;
+      case 5: if constexpr (RB > 5) reg_dense1<V, R, NR, 5>(v, G, jt, jr, sw); break;
+      default: break;
+    }
+  } else {  // GK_DENSE2 / GK_SWAP2
+    const bool sw = kind == GK_SWAP2;
+    switch (G.p0) {
+      case 0: reg_dense2_p1<V, R, NR, RB, 0>(v, G, jt, jr, sw); break;
+      case 1: if constexpr (RB > 1) reg_dense2_p1<V, R, NR, RB, 1>(v, G, jt, jr, sw); break;
+      case 2: if constexpr (RB > 2) reg_dense2_p1<V, R, NR, RB, 2>(v, G, jt, jr, sw); break;
+      case 3: if constexpr (RB > 3) reg_dense2_p1<V, R, NR, RB, 3>(v, G, jt, jr, sw); break;
+      case 4: if constexpr (RB > 4) reg_dense2_p1<V, R, NR, RB, 4>(v, G, jt, jr, sw); break;
+      case 5: if constexpr (RB > 5) reg_dense2_p1<V, R, NR, RB, 5>(v, G, jt, jr, sw); break;
+      default: break;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t pdep32(uint32_t x, uint32_t mask) {
+  uint32_t r = 0;
+  for (uint32_t bit = 1; mask; bit <<= 1) {
+    const uint32_t low = mask & (~mask + 1);
+    if (x & bit) r |= low;
+    mask &= mask - 1;
+  }
+  return r;
+}
+
+template <int NR>
+__device__ __forceinline__ void reg_layout(uint32_t regmask, uint32_t inner, uint32_t* jt, uint32_t* jr) {
+  *jt = pdep32(threadIdx.x, inner & ~regmask);
+  uint32_t bits[6] = {0, 0, 0, 0, 0, 0};
+  uint32_t m = regmask;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    bits[i] = m & (~m + 1);
+    m &= m - 1;
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      if (r & (1 << i)) x |= bits[i];
+    jr[r] = x;
+  }
+}
+
+// Plain complex pair for register arrays: HIP's union-based double2 keeps
+// the compiler from splitting a 32-entry array into registers (it lands in
+// scratch); this struct has the same size and alignment and does split.
+template <typename R>
+struct alignas(2 * sizeof(R)) cplx {
+  R x, y;
+};
+
+template <typename R, int TB, int RB, bool NT>
+__global__ __launch_bounds__(1 << TB) void k_part_reg(typename VecT<R>::type* __restrict__ a_vec, uint64_t nbases,
+                                                      uint64_t outer_mask, uint64_t part_mask,
+                                                      const RegSeg* __restrict__ segs, int nseg,
+                                                      const RegGate* __restrict__ gates) {
+  using V = cplx<R>;
+  V* __restrict__ a = reinterpret_cast<V*>(a_vec);
+  constexpr int NR = 1 << RB;
+  constexpr int K = TB + RB;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* lds = reinterpret_cast<V*>(smem_raw);
+  __shared__ uint64_t tlo[64];
+  __shared__ uint64_t thi[1 << (K > 6 ? K - 6 : 0)];
+  constexpr int KLO = K < 6 ? K : 6;
+  for (int j = threadIdx.x; j < 64; j += (1 << TB)) tlo[j] = j < (1 << KLO) ? pdep64((uint64_t)j, part_mask) : 0;
+  for (int j = threadIdx.x; j < (1 << (K - KLO)); j += (1 << TB)) thi[j] = pdep64((uint64_t)j << KLO, part_mask);
+  __syncthreads();
+  const uint32_t inner = (1u << K) - 1;
+  // The first layout is fixed, so the next base's values can be fetched into
+  // `nxt` while the current base is processed: its loads are in flight
+  // during this base's gates and stores.
+  uint32_t jt0, jr0[NR];
+  reg_layout<NR>(segs[0].regmask, inner, &jt0, jr0);
+  V nxt[NR];
+  if (blockIdx.x < nbases) {
+    const uint64_t base = pdep64(blockIdx.x, outer_mask);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const uint32_t j = jt0 | jr0[r];
+      vcopy(nxt[r], ldg<NT>(&a[base + tlo[j & 63] + thi[j >> 6]]));
+    }
+  }
+  for (uint64_t w = blockIdx.x; w < nbases; w += gridDim.x) {
+    const uint64_t base = pdep64(w, outer_mask);
+    V v[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) vcopy(v[r], nxt[r]);
+    const uint64_t wn = w + gridDim.x;
+    if (wn < nbases) {
+      const uint64_t nbase = pdep64(wn, outer_mask);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const uint32_t j = jt0 | jr0[r];
+        vcopy(nxt[r], ldg<NT>(&a[nbase + tlo[j & 63] + thi[j >> 6]]));
+      }
+    }
+    uint32_t jt = jt0, jr[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) jr[r] = jr0[r];
+    for (int s = 0; s < nseg; ++s) {
+      if (s > 0) {
+        __syncthreads();  // everyone is done reading the previous exchange
+#pragma unroll
+        for (int r = 0; r < NR; ++r) vcopy(lds[jt | jr[r]], v[r]);
+        __syncthreads();
+        reg_layout<NR>(segs[s].regmask, inner, &jt, jr);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) vcopy(v[r], lds[jt | jr[r]]);
+      }
+      const int g1 = segs[s].first + segs[s].count;
+      // the next gate's record is fetched while the current one is applied
+      RegGate cur;
+      if (segs[s].first < g1) cur = gates[segs[s].first];
+      for (int gi = segs[s].first; gi < g1; ++gi) {
+        RegGate nxtg;
+        if (gi + 1 < g1) nxtg = gates[gi + 1];
+        reg_apply<V, R, NR, RB>(v, cur, jt, jr);
+        cur = nxtg;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const uint32_t j = jt | jr[r];
+      stg<NT>(&a[base + tlo[j & 63] + thi[j >> 6]], v[r]);
+    }
   }
 }
 

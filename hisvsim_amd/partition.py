@@ -61,11 +61,17 @@ def nat_part_ids(gates: Sequence[Gate], k: int) -> List[int]:
     return ids
 
 
-def greedy_parts(gates: Sequence[Gate], k: int) -> List[List[Gate]]:
+def greedy_parts(gates: Sequence[Gate], k: int, always: Sequence[int] = ()) -> List[List[Gate]]:
+    """``always``: qubits every part includes anyway (the device adds the
+    lowest qubits to each part for contiguous HBM runs); they count towards k
+    from the start, so gates on them join parts for free."""
     remaining = list(range(len(gates)))
     parts: List[List[Gate]] = []
+    base_q = set(always)
+    if len(base_q) > k:
+        raise ValueError("more always-included qubits than the part width")
     while remaining:
-        part_q: set = set()
+        part_q: set = set(base_q)
         blocked: set = set()
         taken: List[int] = []
         keep: List[int] = []

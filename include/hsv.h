@@ -131,6 +131,16 @@ int hsv_gate_matrix(const hsv_gate* g, double* out, int* dim);
 int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int* qubits_out,
                       int* num_part_qubits, double* u_out, double* unitarity_err);
 
+/* Kernel-shape knobs for the autotuner (not in the reference):
+ *   "part_kernel" 1 = register-blocked part kernel for parts of >= 8 qubits,
+ *                 0 = LDS kernel for every part;
+ *   "reg_bits"    preferred values per thread of the register kernel, as
+ *                 log2 (2..5; 0 = default for the dtype);
+ *   "reg_waves"   persistent workgroups per CU and occupancy slot (1..64);
+ *   "low_slots"   lowest qubits added to every LDS-staged part for
+ *                 contiguous HBM runs (0..6). */
+int hsv_tune(hsv_state* s, const char* key, int value);
+
 /* Waits for all work queued on the handle. */
 int hsv_synchronize(hsv_state* s);
 /* Launch timing: while on, every apply/permute call records a HIP event pair

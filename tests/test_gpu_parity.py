@@ -258,3 +258,42 @@ def test_errors_are_reported():
             s.apply_gate(Gate("SWAP", (1,)))
         with pytest.raises(capi.HsvError, match="ARG"):
             s.permute_qubits([0, 0, 1, 2, 3, 4])
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("k,reg_bits", [(8, 0), (9, 2), (10, 3), (11, 4), (12, 0), (13, 4), (13, 5), (12, 5)])
+def test_register_part_kernel(orc, dtype, k, reg_bits):
+    """Parts of >= 8 qubits (with the low slots) run on the register-blocked
+    kernel; every gate type, controls on register and thread bits."""
+    n = 16
+    rng = np.random.default_rng(1000 + 17 * k + reg_bits)
+    names = [g for g in ALL_ONE + ALL_TWO if g != "I"]
+    parts = random_parts(rng, n, 5, k, 4 * k, names=names, max_controls=2)
+    a = rand_state(n, k)
+    want = a.copy()
+    for p in parts:
+        orc.apply_gates(want, p)
+    with capi.State(n, dtype) as s:
+        s.tune("reg_bits", reg_bits)
+        s.tune("low_slots", 0)
+        s.write(a)
+        for p in parts:
+            s.apply_part(p, 0)
+        got = s.read()
+    # relative to the largest amplitude: C1/C2 are not unitary and grow the norm
+    scale = max(1.0, np.abs(want).max())
+    tol = (TOL if dtype == "complex128" else 1e-5) * scale
+    assert np.abs(got - want).max() <= tol
+
+
+def test_register_and_lds_kernels_agree():
+    n = 20
+    gates = random_layered(n, 6, seed=8)
+    from hisvsim_amd.partition import greedy_parts
+    parts = greedy_parts(gates, 9)
+    with capi.State(n) as a, capi.State(n) as b:
+        b.tune("part_kernel", 0)
+        for p in parts:
+            a.apply_part(p, 0)
+            b.apply_part(p, 0)
+        assert a.max_abs_diff(b) <= TOL
