@@ -100,6 +100,14 @@ SIGNATURES = {
     "hsv_fused_unitary": (ctypes.c_int, [ctypes.c_int, _G, ctypes.c_int, _I, _I, _D, _D]),
     "hsv_synchronize": (ctypes.c_int, [_P]),
     "hsv_tune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
+    "hsv_qaoa_create": (ctypes.c_int, [ctypes.c_int, _I, _I, _I, _I, ctypes.POINTER(_P)]),
+    "hsv_qaoa_destroy": (ctypes.c_int, [_P]),
+    "hsv_qaoa_set_angles": (ctypes.c_int, [_P, _D, _D]),
+    "hsv_qaoa_run": (ctypes.c_int, [_P, _D]),
+    "hsv_qaoa_tune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
+    "hsv_qaoa_last_ms": (ctypes.c_int, [_P, _D]),
+    "hsv_qaoa_read_state": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "hsv_qaoa_total_amplitudes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "hsv_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "hsv_kernel_times": (ctypes.c_int, [_P, _D, ctypes.c_int, _I]),
     "hsv_set_opt_slots": (None, [ctypes.c_int]),

@@ -41,6 +41,8 @@ int fail(int code, const std::string& msg) {
 
 }  // namespace
 
+void hsv_set_last_error(const std::string& msg) { g_err = msg; }
+
 struct hsv_state {
   int n = 0;
   int dtype = HSV_C128;

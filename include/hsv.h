@@ -141,6 +141,30 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *                 contiguous HBM runs (0..6). */
 int hsv_tune(hsv_state* s, const char* key, int value);
 
+/* ---- Batched QAOA MaxCut (not in the reference) ------------------------
+ * Many instances of uneven qubit counts in one batch: instance i has
+ * num_qubits[i] qubits, edges[2*e], edges[2*e+1] for e in
+ * [edge_offsets[i], edge_offsets[i+1]), and layers[i] QAOA layers. The state
+ * is complex64 per instance; the circuit is H on every qubit, then per layer
+ * ZZ(gamma) = exp(-i gamma/2 Z_u Z_v) on every edge (the reference's RZZ,
+ * basic_gates.hpp:1249-1252) and RX(2 beta) on every qubit. Expectations
+ * <C>, C = sum_edges (1 - Z_u Z_v)/2, are FP64 reductions. The batch runs
+ * as one hipGraph (captured on first run, replayed after); angles are
+ * per instance and layer, flattened in instance order. */
+typedef struct hsv_qaoa_batch hsv_qaoa_batch;
+int hsv_qaoa_create(int num_instances, const int* num_qubits, const int* edge_offsets, const int* edges,
+                    const int* layers, hsv_qaoa_batch** out);
+int hsv_qaoa_destroy(hsv_qaoa_batch* b);
+int hsv_qaoa_set_angles(hsv_qaoa_batch* b, const double* gammas, const double* betas);
+int hsv_qaoa_run(hsv_qaoa_batch* b, double* expectations);
+/* Tile shape for the autotuner: "chunk_bits" (8..13), "threads"
+ * (128/256/512), "grid_per_cu" (1..64). Invalidates the captured graph. */
+int hsv_qaoa_tune(hsv_qaoa_batch* b, const char* key, int value);
+/* Device time (ms) of the last hsv_qaoa_run, from HIP events. */
+int hsv_qaoa_last_ms(hsv_qaoa_batch* b, double* ms);
+int hsv_qaoa_read_state(hsv_qaoa_batch* b, int instance, double* amps);
+int hsv_qaoa_total_amplitudes(hsv_qaoa_batch* b, uint64_t* out);
+
 /* Waits for all work queued on the handle. */
 int hsv_synchronize(hsv_state* s);
 /* Launch timing: while on, every apply/permute call records a HIP event pair
