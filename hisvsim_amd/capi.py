@@ -100,6 +100,17 @@ SIGNATURES = {
     "hsv_fused_unitary": (ctypes.c_int, [ctypes.c_int, _G, ctypes.c_int, _I, _I, _D, _D]),
     "hsv_synchronize": (ctypes.c_int, [_P]),
     "hsv_tune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
+    "hsv_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "hsv_mlp_destroy": (ctypes.c_int, [_P]),
+    "hsv_mlp_num_params": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "hsv_mlp_get_params": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
+    "hsv_mlp_set_params": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
+    "hsv_mlp_forward": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_float)]),
+    "hsv_mlp_gradient": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "hsv_mlp_train": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
     "hsv_qaoa_create": (ctypes.c_int, [ctypes.c_int, _I, _I, _I, _I, ctypes.POINTER(_P)]),
     "hsv_qaoa_destroy": (ctypes.c_int, [_P]),
     "hsv_qaoa_set_angles": (ctypes.c_int, [_P, _D, _D]),

@@ -1,0 +1,198 @@
+"""Learned cost model: a small MLP that predicts a part's kernel time from its
+features, trained on the GPU (libhsv's hsv_mlp_* kernels) on per-part timings
+the simulator records itself. The planner uses it to choose the part width k
+and the partition; the plain partitioner (natural order, the reference's
+``*_part_nat`` method) is the fallback, and measured runtime judges the
+model.
+
+Features of a part, for a state of n qubits and a kernel mode (LDS/register
+part kernel or fused FP64 unitary):
+  part qubits k, the width the device runs it at (k plus the low qubits it
+  adds), gate count, 1-qubit and multi-qubit gate counts, the multi-qubit
+  fraction, min / max / spread of the qubit indices, whether it touches the
+  top 3 (high-order) qubits, the mode and n.
+Target: log(ms per pass scaled to a 2^30-amplitude state), so one model
+covers every n (a pass is memory-bound work proportional to 2^n).
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import os
+import time
+from typing import Callable, Dict, List, Sequence, Tuple
+
+import numpy as np
+
+from . import capi
+from .circuit import Gate
+from .partition import greedy_parts, nat_parts
+
+MODES = ("part", "fused64")
+LOW_SLOTS = 5
+N_FEATURES = 12
+PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "costmodel_params.json")
+
+
+def part_features(part: Sequence[Gate], n: int, mode: str) -> np.ndarray:
+    qs = sorted({q for g in part for q in g.qubits()})
+    k = len(qs)
+    if mode == "part":
+        width = len(set(qs) | set(range(min(LOW_SLOTS, n))))
+    else:
+        width = max(4, k)
+    n1 = sum(1 for g in part if len(g.qubits()) == 1)
+    n2 = len(part) - n1
+    return np.array([
+        k / 10.0, width / 10.0, len(part) / 20.0, n1 / 20.0, n2 / 20.0, n2 / max(1, len(part)),
+        qs[0] / 30.0, qs[-1] / 30.0, (qs[-1] - qs[0]) / 30.0, float(qs[-1] >= n - 3),
+        float(mode == "fused64"), (n - 25) / 5.0,
+    ], dtype=np.float32)
+
+
+def target_of(ms: float, n: int) -> float:
+    return math.log(ms / 2.0 ** (n - 30))
+
+
+def ms_of(target: float, n: int) -> float:
+    return math.exp(target) * 2.0 ** (n - 30)
+
+
+class MLP:
+    """The cost-model network on the GPU."""
+
+    def __init__(self, in_features: int = N_FEATURES, hidden: int = 64, seed: int = 1):
+        self.in_features, self.hidden = in_features, hidden
+        self._h = ctypes.c_void_p()
+        capi._check(capi.lib().hsv_mlp_create(in_features, hidden, seed, ctypes.byref(self._h)))
+        c = ctypes.c_uint64()
+        capi._check(capi.lib().hsv_mlp_num_params(self._h, ctypes.byref(c)))
+        self.num_params = c.value
+
+    def close(self):
+        if self._h:
+            capi.lib().hsv_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _fp(a: np.ndarray):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.num_params, np.float32)
+        capi._check(capi.lib().hsv_mlp_get_params(self._h, self._fp(out)))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params
+        capi._check(capi.lib().hsv_mlp_set_params(self._h, self._fp(p)))
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(len(X), np.float32)
+        capi._check(capi.lib().hsv_mlp_forward(self._h, self._fp(X), len(X), self._fp(out)))
+        return out
+
+    def gradient(self, X: np.ndarray, y: np.ndarray) -> Tuple[float, np.ndarray]:
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(self.num_params, np.float32)
+        loss = ctypes.c_float()
+        capi._check(capi.lib().hsv_mlp_gradient(self._h, self._fp(X), self._fp(y), len(X), ctypes.byref(loss),
+                                                self._fp(g)))
+        return loss.value, g
+
+    def train(self, X: np.ndarray, y: np.ndarray, epochs: int = 200, batch: int = 512, lr: float = 3e-3,
+              seed: int = 0) -> float:
+        rng = np.random.default_rng(seed)
+        perm = rng.permutation(len(X))
+        X = np.ascontiguousarray(X[perm], np.float32)
+        y = np.ascontiguousarray(y[perm], np.float32)
+        loss = ctypes.c_float()
+        capi._check(capi.lib().hsv_mlp_train(self._h, self._fp(X), self._fp(y), len(X), epochs, batch, lr,
+                                             ctypes.byref(loss)))
+        return loss.value
+
+    def save(self, path: str = PARAMS_FILE, meta: dict | None = None) -> None:
+        with open(path, "w") as f:
+            json.dump({"in": self.in_features, "hidden": self.hidden, "params": self.params().tolist(),
+                       "meta": meta or {}}, f)
+
+    @classmethod
+    def load(cls, path: str = PARAMS_FILE) -> "MLP":
+        with open(path) as f:
+            d = json.load(f)
+        m = cls(d["in"], d["hidden"])
+        m.set_params(np.array(d["params"], np.float32))
+        return m
+
+
+# ---- planning ------------------------------------------------------------------
+def candidate_specs() -> List[Tuple[str, int, str]]:
+    """(mode, k, partitioner) candidates: the fused unitary at k = 5..7 and
+    the part kernel at k = 8..13 (its parts already hold the 5 low qubits),
+    with the DAG-aware greedy or the natural-order partitioner (k = 5..10 for
+    natural order on the part kernel). k is the block width the model picks."""
+    specs = [("fused64", k, name) for k in range(5, 8) for name in ("greedy", "nat")]
+    specs += [("part", k, "greedy") for k in range(8, 14)]
+    specs += [("part", k, "nat") for k in range(5, 11)]
+    return specs
+
+
+def build_plan(gates: Sequence[Gate], n: int, spec: Tuple[str, int, str]) -> dict:
+    mode, k, name = spec
+    if name == "nat":
+        parts = nat_parts(gates, k)
+    elif mode == "part":
+        parts = greedy_parts(gates, max(k, min(LOW_SLOTS, n)), always=range(min(LOW_SLOTS, n)))
+    else:
+        parts = greedy_parts(gates, k)
+    return {"mode": mode, "k": k, "partitioner": name, "parts": parts}
+
+
+def candidate_plans(gates: Sequence[Gate], n: int) -> List[dict]:
+    return [build_plan(gates, n, s) for s in candidate_specs()]
+
+
+def fallback_plan(gates: Sequence[Gate], n: int, k: int = 10) -> dict:
+    """The plain partitioner: natural order, k = 10, on the part kernel."""
+    return {"mode": "part", "k": k, "partitioner": "nat", "parts": nat_parts(gates, k)}
+
+
+def predict_plan_ms(model: MLP, plan: dict, n: int) -> float:
+    X = np.stack([part_features(p, n, plan["mode"]) for p in plan["parts"]])
+    return float(sum(ms_of(float(t), n) for t in model.predict(X)))
+
+
+def choose_plan(model: MLP, gates: Sequence[Gate], n: int) -> Tuple[dict, List[dict]]:
+    plans = candidate_plans(gates, n)
+    for p in plans:
+        p["predicted_ms"] = predict_plan_ms(model, p, n)
+    best = min(plans, key=lambda p: p["predicted_ms"])
+    return best, plans
+
+
+def run_plan(s: capi.State, plan: dict, timed: bool = True) -> np.ndarray:
+    """Runs a plan on the state; returns each part's kernel time (ms)."""
+    if timed:
+        s.set_timing(True)
+        s.kernel_times()
+    for p in plan["parts"]:
+        if plan["mode"] == "fused64":
+            s.apply_fused(p, "f64")
+        else:
+            s.apply_part(p, 0)
+    s.synchronize()
+    if not timed:
+        return np.zeros(0)
+    t = s.kernel_times()
+    s.set_timing(False)
+    return t

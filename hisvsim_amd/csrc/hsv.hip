@@ -343,8 +343,8 @@ bool reg_shape_available(int tb, int rb, int elem) {
   if (elem == 16 && rb > 4) return false;
   switch (tb) {
     case 6: return rb >= 2 && rb <= 5;
-    case 7: case 8: return rb == 4 || rb == 5;
-    case 9: return rb == 4;
+    case 7: case 8: return rb >= 3 && rb <= 5;
+    case 9: return rb == 3 || rb == 4;
     default: return false;
   }
 }
@@ -410,6 +410,7 @@ int launch_reg_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t par
 #define HSV_REG_CASE(T_, R_) \
   if (TB == T_ && RB == R_) return launch_reg<R, T_, R_>(s, P, nbases, outer, part_mask);
   HSV_REG_CASE(6, 2) HSV_REG_CASE(6, 3) HSV_REG_CASE(6, 4)
+  HSV_REG_CASE(7, 3) HSV_REG_CASE(8, 3) HSV_REG_CASE(9, 3)
   HSV_REG_CASE(7, 4) HSV_REG_CASE(8, 4) HSV_REG_CASE(9, 4)
   if constexpr (sizeof(R) == 4) {
     HSV_REG_CASE(6, 5) HSV_REG_CASE(7, 5) HSV_REG_CASE(8, 5)

@@ -86,6 +86,9 @@ def greedy_parts(gates: Sequence[Gate], k: int, always: Sequence[int] = ()) -> L
             else:
                 blocked |= gq
                 keep.append(i)
+        if not taken:
+            raise ValueError(f"gate {gates[remaining[0]]} does not fit a part of {k} qubits "
+                             f"that already holds {sorted(base_q)}")
         parts.append([gates[i] for i in taken])
         remaining = keep
     return parts

@@ -1,0 +1,140 @@
+#!/usr/bin/env python3
+"""Trains and evaluates the cost model on the GPU.
+
+    python -m hisvsim_amd.train_costmodel [--circuits 2000] [--out DIR]
+
+1. Data: random layered circuits (U3 + random-matching CZ, the benchmark
+   generator) with n ~ U{20..30} qubits and depth ~ U{4..10}. Each runs under
+   two plans drawn from the candidate set (fused FP64 at k = 5..10, the part
+   kernel at k = 8..13, greedy or natural-order parts); every part's kernel
+   time is recorded with HIP events.
+2. Training: the 3-layer MLP (64 hidden units, FP32) on log(ms per 2^30
+   amplitudes), by libhsv's own forward/backward/Adam kernels; 10% held out.
+3. Evaluation: 32-qubit random layered circuits of depth ~ U{30..60}. The
+   model predicts every candidate plan, picks the cheapest, and the picked
+   plan and the fallback (natural-order parts, k = 10) are both run.
+   Reported: per-part and per-plan predicted-vs-measured error and the
+   end-to-end times.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import capi
+from .circuit import random_layered
+from .costmodel import (MLP, PARAMS_FILE, build_plan, candidate_specs, choose_plan, fallback_plan, ms_of, part_features,
+                        predict_plan_ms, run_plan, target_of)
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10)):
+    rng = np.random.default_rng(seed)
+    X, y, meta = [], [], []
+    t0 = time.time()
+    states = {}
+    for c in range(num):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        gates = random_layered(n, depth, seed=int(rng.integers(1 << 30)))
+        specs = candidate_specs()
+        if n not in states:
+            for s in states.values():
+                s.close()
+            states = {n: capi.State(n)}
+        s = states[n]
+        for pi in rng.choice(len(specs), size=2, replace=False):
+            plan = build_plan(gates, n, specs[int(pi)])
+            s.reset()
+            times = run_plan(s, plan)
+            for part, ms in zip(plan["parts"], times):
+                X.append(part_features(part, n, plan["mode"]))
+                y.append(target_of(float(ms), n))
+            meta.append({"n": n, "depth": depth, "mode": plan["mode"], "k": plan["k"],
+                         "partitioner": plan["partitioner"], "parts": len(plan["parts"]),
+                         "ms": float(times.sum())})
+        if (c + 1) % 100 == 0:
+            log(f"[collect] {c + 1}/{num} circuits, {len(X)} parts, {time.time() - t0:.0f} s")
+    for s in states.values():
+        s.close()
+    return np.array(X, np.float32), np.array(y, np.float32), meta
+
+
+def evaluate(model: MLP, num: int, seed: int, n: int = 32):
+    rng = np.random.default_rng(seed)
+    rows = []
+    part_err = []
+    with capi.State(n) as s:
+        for c in range(num):
+            depth = int(rng.integers(30, 61))
+            gates = random_layered(n, depth, seed=int(rng.integers(1 << 30)))
+            best, plans = choose_plan(model, gates, n)
+            fb = fallback_plan(gates, n)
+            fb["predicted_ms"] = predict_plan_ms(model, fb, n)
+            res = {"depth": depth, "gates": len(gates)}
+            for name, plan in (("model", best), ("fallback", fb)):
+                s.reset()
+                t0 = time.perf_counter()
+                times = run_plan(s, plan)
+                wall = time.perf_counter() - t0
+                Xp = np.stack([part_features(p, n, plan["mode"]) for p in plan["parts"]])
+                pred = np.array([ms_of(float(t), n) for t in model.predict(Xp)])
+                part_err.extend(list(np.abs(pred - times) / times))
+                res[name] = {"mode": plan["mode"], "k": plan["k"], "partitioner": plan["partitioner"],
+                             "parts": len(plan["parts"]), "predicted_ms": plan["predicted_ms"],
+                             "measured_kernel_ms": float(times.sum()), "wall_s": wall}
+                log(f"[eval] circuit {c} depth {depth} {name}: {res[name]}")
+            res["speedup"] = res["fallback"]["measured_kernel_ms"] / res["model"]["measured_kernel_ms"]
+            ranked = sorted(plans, key=lambda p: p["predicted_ms"])[:5]
+            res["top_predicted"] = [{"mode": p["mode"], "k": p["k"], "partitioner": p["partitioner"],
+                                     "parts": len(p["parts"]), "predicted_ms": p["predicted_ms"]} for p in ranked]
+            rows.append(res)
+    return rows, float(np.median(part_err)), float(np.mean(part_err))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--circuits", type=int, default=2000)
+    ap.add_argument("--eval", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--out", default="gpurun_out")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+    os.makedirs(args.out, exist_ok=True)
+    X, y, meta = collect(args.circuits, args.seed)
+    log(f"[train] {len(X)} samples")
+    rng = np.random.default_rng(1)
+    idx = rng.permutation(len(X))
+    nval = max(1, len(X) // 10)
+    val, tr = idx[:nval], idx[nval:]
+    model = MLP(seed=7)
+    t0 = time.time()
+    loss = model.train(X[tr], y[tr], epochs=args.epochs, batch=512, lr=2e-3)
+    train_s = time.time() - t0
+    pv = model.predict(X[val])
+    val_mse = float(np.mean((pv - y[val]) ** 2))
+    val_rel = float(np.median(np.abs(np.exp(pv - y[val]) - 1)))
+    log(f"[train] loss {loss:.4f}, held-out MSE {val_mse:.4f}, median relative error {val_rel:.3f}, {train_s:.1f} s")
+    rows, med, mean = evaluate(model, args.eval, args.seed + 1)
+    report = {"samples": len(X), "circuits": args.circuits, "train_loss": loss, "val_mse_log": val_mse,
+              "val_median_rel_err": val_rel, "train_seconds": train_s,
+              "eval_part_median_rel_err": med, "eval_part_mean_rel_err": mean, "eval": rows,
+              "collect_plans": meta[:50]}
+    with open(os.path.join(args.out, "costmodel_report.json"), "w") as f:
+        json.dump(report, f, indent=1)
+    model.save(os.path.join(args.out, "costmodel_params.json"),
+               meta={"samples": len(X), "val_median_rel_err": val_rel})
+    log(json.dumps({k: v for k, v in report.items() if k not in ("eval", "collect_plans")}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -165,6 +165,24 @@ int hsv_qaoa_last_ms(hsv_qaoa_batch* b, double* ms);
 int hsv_qaoa_read_state(hsv_qaoa_batch* b, int instance, double* amps);
 int hsv_qaoa_total_amplitudes(hsv_qaoa_batch* b, uint64_t* out);
 
+/* ---- Cost-model MLP (not in the reference) -------------------------------
+ * in -> hidden -> hidden -> 1, ReLU, FP32, trained on the GPU (MSE, Adam) by
+ * this library's forward/backward kernels. Parameters are one flat array:
+ * W1[in][H] | b1[H] | W2[H][H] | b2[H] | W3[H] | b3. X is row-major
+ * [n][in]. Deterministic. */
+typedef struct hsv_mlp hsv_mlp;
+int hsv_mlp_create(int in_features, int hidden, uint64_t seed, hsv_mlp** out);
+int hsv_mlp_destroy(hsv_mlp* m);
+int hsv_mlp_num_params(hsv_mlp* m, uint64_t* count);
+int hsv_mlp_get_params(hsv_mlp* m, float* out);
+int hsv_mlp_set_params(hsv_mlp* m, const float* in);
+int hsv_mlp_forward(hsv_mlp* m, const float* X, int n, float* y_out);
+/* Loss (mean squared error) and, if grads_out != NULL, all gradients. */
+int hsv_mlp_gradient(hsv_mlp* m, const float* X, const float* target, int n, float* loss, float* grads_out);
+/* `epochs` passes of Adam over mini-batches of `batch` samples in order. */
+int hsv_mlp_train(hsv_mlp* m, const float* X, const float* target, int n, int epochs, int batch, float lr,
+                  float* loss);
+
 /* Waits for all work queued on the handle. */
 int hsv_synchronize(hsv_state* s);
 /* Launch timing: while on, every apply/permute call records a HIP event pair

@@ -1,0 +1,77 @@
+"""Cost-model MLP: the numpy restatement pinned by finite differences (CPU),
+the GPU forward/backward kernels against it, and training behaviour (GPU)."""
+import numpy as np
+import pytest
+
+from hisvsim_amd.circuit import random_layered
+from hisvsim_amd.costmodel import N_FEATURES, part_features, candidate_plans, fallback_plan
+from hisvsim_amd.partition import check_parts, greedy_parts
+from oracle import mlp_ref
+
+IN, H = 6, 16
+
+
+def rand_params(rng, n_in=IN, h=H):
+    P = n_in * h + h + h * h + h + h + 1
+    return rng.normal(0, 0.4, P)
+
+
+def test_reference_gradient_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    p = rand_params(rng)
+    X = rng.normal(size=(20, IN))
+    y = rng.normal(size=20)
+    _, g = mlp_ref.loss_and_grad(p, X, y, IN, H)
+    eps = 1e-6
+    for i in rng.choice(len(p), 40, replace=False):
+        d = np.zeros_like(p)
+        d[i] = eps
+        lp, _ = mlp_ref.loss_and_grad(p + d, X, y, IN, H)
+        lm, _ = mlp_ref.loss_and_grad(p - d, X, y, IN, H)
+        assert abs((lp - lm) / (2 * eps) - g[i]) <= 1e-6 * max(1.0, abs(g[i]))
+
+
+def test_features_and_candidates():
+    n = 24
+    gates = random_layered(n, 6, seed=2)
+    parts = greedy_parts(gates, 8)
+    f = part_features(parts[0], n, "part")
+    assert f.shape == (N_FEATURES,) and np.all(np.isfinite(f))
+    plans = candidate_plans(gates, n)
+    assert {p["mode"] for p in plans} == {"part", "fused64"}
+    for p in plans:
+        check_parts(gates, p["parts"])
+    check_parts(gates, fallback_plan(gates, n)["parts"], 10)
+
+
+@pytest.mark.gpu
+def test_gpu_forward_and_gradient_match_reference():
+    from hisvsim_amd.costmodel import MLP
+    rng = np.random.default_rng(1)
+    m = MLP(IN, H, seed=3)
+    p = m.params().astype(np.float64)
+    X = rng.normal(size=(300, IN)).astype(np.float32)
+    y = rng.normal(size=300).astype(np.float32)
+    want, _ = mlp_ref.forward(p, X, IN, H)
+    assert np.abs(m.predict(X) - want).max() <= 1e-4 * max(1.0, np.abs(want).max())
+    loss, g = m.gradient(X, y)
+    rl, rg = mlp_ref.loss_and_grad(p, X, y, IN, H)
+    assert abs(loss - rl) <= 1e-4 * rl
+    assert np.abs(g - rg).max() <= 1e-4 * max(1.0, np.abs(rg).max())
+    m.close()
+
+
+@pytest.mark.gpu
+def test_gpu_training_fits_and_is_deterministic():
+    from hisvsim_amd.costmodel import MLP
+    rng = np.random.default_rng(2)
+    X = rng.uniform(-1, 1, size=(4096, N_FEATURES)).astype(np.float32)
+    y = (np.sin(2 * X[:, 0]) + X[:, 1] * X[:, 2] + 0.5 * X[:, 3]).astype(np.float32)
+    a, b = MLP(seed=5), MLP(seed=5)
+    l0, _ = a.gradient(X, y)
+    la = a.train(X, y, epochs=60, batch=256, lr=3e-3)
+    lb = b.train(X, y, epochs=60, batch=256, lr=3e-3)
+    assert la < 0.1 * l0
+    assert la == lb and np.array_equal(a.params(), b.params())
+    a.close()
+    b.close()
