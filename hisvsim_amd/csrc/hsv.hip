@@ -60,7 +60,7 @@ struct hsv_state {
   int max_lds = 65536;
   int num_cu = 256;
   // tuning knobs (hsv_tune)
-  int reg_waves = 2;                 // persistent-grid factor of the register part kernel
+  int reg_waves = 4;                 // persistent-grid factor of the register part kernel
   int part_kernel = 1;               // 1: register-blocked part kernel for k >= 8; 0: LDS kernel only
   int reg_bits = 0;                  // preferred register bits (0: per dtype)
   // Slots 0..low_slots-1 join every LDS-staged part when they fit: gathered

@@ -1,0 +1,135 @@
+#!/usr/bin/env python3
+"""QASM driver: parse a circuit, partition it, run it on the GPU, measure.
+
+Counterpart of the reference driver's main flow and part executor
+(qasm_assembler_standalone.py:3765-3870 and ``execute_qasm_parts`` :1768-2030,
+``execute_qasm`` :1377-1617 for the per-gate path):
+
+    python -m hisvsim_amd.driver --app circuit.qasm [--parts --partfile F]
+        [--nat K | --greedy K] [--slots] [--mode part|fused64|fused32|pergate]
+        [--lowering reference|textbook] [--dot] [--dump] [--shots N]
+        [--n T --alloc A --initn T]   (accepted for compatibility; no effect)
+
+Parts come from a parts file (``<order> <node> <part>`` lines, the reference
+format), or from this package's natural-order (``--nat K``, the reference's
+``*_part_nat`` method) or DAG-aware greedy partitioner (``--greedy K``).
+Without ``--parts`` every gate is applied on its own (the per-gate path).
+``--slots`` adds the fast low qubits to every part (``set_opt_slots(1)``).
+``--dot`` writes the DAG files next to the input, ``--dump`` the
+``<file>_exec`` ordered-gate dump. Measurements in the file are taken at
+the end (deferred): a gate on an already measured qubit is an error. One
+JSON line reports the timings (µs, like the reference's counters) and, per
+measured qubit, P(1) and ``--shots`` samples of the measured register.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from typing import List
+
+import numpy as np
+
+from . import capi, qasm
+from .partition import check_parts, greedy_parts, nat_parts
+
+NUM_MEASURE = 10   # shots per measurement in the reference driver (qasm_assembler_standalone.py:29)
+
+
+def measured_qubits(prog: qasm.QasmProgram) -> List[int]:
+    """Qubit ids measured in the file, in order; raises if a gate acts on a
+    qubit after it was measured (the deferred-measurement condition)."""
+    ids = prog.qubit_ids()
+    done: List[int] = []
+    for op in prog.ops:
+        if op.gate == "measure":
+            q = ids[op.qubits[0]]
+            if q not in done:
+                done.append(q)
+        elif any(ids[x] in done for x in op.qubits):
+            raise qasm.QasmError(f"line {op.index + 1}: gate on a qubit measured earlier "
+                                 f"(mid-circuit measurement is not supported)")
+    return done
+
+
+def plan_parts(prog: qasm.QasmProgram, args) -> List[list]:
+    gates = prog.gates(args.lowering)
+    if args.partfile:
+        parts = qasm.parts_from_parts_file(prog, args.partfile, args.lowering)
+    elif args.nat:
+        parts = nat_parts(gates, args.nat)
+    elif args.greedy:
+        parts = greedy_parts(gates, args.greedy)
+    else:
+        parts = nat_parts(gates, 10)
+    check_parts(gates, parts)
+    return parts
+
+
+def run(args) -> dict:
+    t0 = time.perf_counter()
+    prog = qasm.parse_file(args.app)
+    gates = prog.gates(args.lowering)
+    meas = measured_qubits(prog)
+    t_parse = time.perf_counter() - t0
+    if args.dot:
+        qasm.write_dot_files(prog)
+    if args.dump:
+        with open(prog.filename + "_exec", "w") as f:
+            f.write(qasm.exec_dump(prog))
+    parts = plan_parts(prog, args) if args.parts else []
+    n = prog.num_qubits
+    dtype = "complex128" if args.dtype == "auto" else args.dtype
+    with capi.State(n, dtype) as s:
+        s.synchronize()
+        t1 = time.perf_counter()
+        if not args.parts or args.mode == "pergate":
+            s.apply_gates(gates)
+        elif args.mode == "part":
+            for p in parts:
+                s.apply_part(p, 1 if args.slots else 0)
+        else:
+            for p in parts:
+                s.apply_fused(p, "f64" if args.mode == "fused64" else "f32split")
+        s.synchronize()
+        t_sim = time.perf_counter() - t1
+        probs = {q: s.probability(q, 1) for q in meas}
+        samples = (s.measure(meas, np.random.default_rng(args.seed).random(args.shots))
+                   if meas and args.shots else [])
+        norm2 = s.norm2()
+    return {
+        "app": args.app, "qubits": n, "gates": len(gates), "parts": len(parts) if args.parts else len(gates),
+        "mode": args.mode if args.parts else "pergate", "lowering": args.lowering, "dtype": dtype,
+        "compile_us": int(t_parse * 1e6), "sim_us": int(t_sim * 1e6), "norm2": norm2,
+        "measured": meas, "p1": {str(q): p for q, p in probs.items()},
+        "samples": [list(map(int, x)) for x in samples],
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--app", required=True, help="QASM file")
+    ap.add_argument("--parts", action="store_true", help="run partitioned (else per gate)")
+    ap.add_argument("--partfile", default="", help="parts file (<order> <node> <part> lines)")
+    ap.add_argument("--nat", type=int, default=0, help="natural-order parts of K qubits")
+    ap.add_argument("--greedy", type=int, default=0, help="DAG-aware greedy parts of K qubits")
+    ap.add_argument("--slots", action="store_true", help="add the fast low qubits to every part")
+    ap.add_argument("--mode", default="part", choices=["part", "fused64", "fused32", "pergate"])
+    ap.add_argument("--lowering", default="reference", choices=["reference", "textbook"])
+    ap.add_argument("--dtype", default="auto", choices=["auto", "complex128", "complex64"],
+                    help="state precision (auto: complex128)")
+    ap.add_argument("--dot", action="store_true", help="write <file>_PART_<i>.dot")
+    ap.add_argument("--dump", action="store_true", help="write <file>_exec")
+    ap.add_argument("--shots", type=int, default=NUM_MEASURE)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--n", type=int, default=0, help="host threads (no effect on the GPU path)")
+    ap.add_argument("--alloc", type=int, default=0, help="allocation scheme (no effect on the GPU path)")
+    ap.add_argument("--initn", type=int, default=0, help="init threads (no effect on the GPU path)")
+    args = ap.parse_args(argv)
+    print(json.dumps(run(args)), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

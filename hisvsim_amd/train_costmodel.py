@@ -5,9 +5,9 @@
 
 1. Data: random layered circuits (U3 + random-matching CZ, the benchmark
    generator) with n ~ U{20..30} qubits and depth ~ U{4..10}. Each runs under
-   two plans drawn from the candidate set (fused FP64 at k = 5..10, the part
-   kernel at k = 8..13, greedy or natural-order parts); every part's kernel
-   time is recorded with HIP events.
+   two plans drawn from the candidate set (fused FP64 at k = 5..7, the part
+   kernel at k = 8..13 with greedy parts or k = 5..10 with natural-order
+   parts); every part's kernel time is recorded with HIP events.
 2. Training: the 3-layer MLP (64 hidden units, FP32) on log(ms per 2^30
    amplitudes), by libhsv's own forward/backward/Adam kernels; 10% held out.
 3. Evaluation: 32-qubit random layered circuits of depth ~ U{30..60}. The
@@ -36,23 +36,33 @@ def log(msg):
     print(msg, flush=True)
 
 
-def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10)):
+def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10), max_seconds: float = 0.0):
+    """Circuits are drawn first and run in blocks of 100, each block in
+    ascending n (one state allocation per n and block). With max_seconds > 0
+    collection stops at the first block boundary past that time."""
     rng = np.random.default_rng(seed)
-    X, y, meta = [], [], []
-    t0 = time.time()
-    states = {}
-    for c in range(num):
+    jobs = []
+    specs = candidate_specs()
+    for _ in range(num):
         n = int(rng.integers(n_range[0], n_range[1] + 1))
         depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
-        gates = random_layered(n, depth, seed=int(rng.integers(1 << 30)))
-        specs = candidate_specs()
-        if n not in states:
-            for s in states.values():
+        picks = [int(i) for i in rng.choice(len(specs), size=2, replace=False)]
+        jobs.append((n, depth, int(rng.integers(1 << 30)), picks))
+    jobs = [j for b in range(0, num, 100) for j in sorted(jobs[b:b + 100], key=lambda j: j[0])]
+    X, y, meta = [], [], []
+    t0 = time.time()
+    s = None
+    for c, (n, depth, gseed, picks) in enumerate(jobs):
+        if max_seconds > 0 and c % 100 == 0 and c and time.time() - t0 > max_seconds:
+            log(f"[collect] time budget reached after {c} circuits")
+            break
+        gates = random_layered(n, depth, seed=gseed)
+        if s is None or s.n != n:
+            if s is not None:
                 s.close()
-            states = {n: capi.State(n)}
-        s = states[n]
-        for pi in rng.choice(len(specs), size=2, replace=False):
-            plan = build_plan(gates, n, specs[int(pi)])
+            s = capi.State(n)
+        for pi in picks:
+            plan = build_plan(gates, n, specs[pi])
             s.reset()
             times = run_plan(s, plan)
             for part, ms in zip(plan["parts"], times):
@@ -61,9 +71,9 @@ def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10)):
             meta.append({"n": n, "depth": depth, "mode": plan["mode"], "k": plan["k"],
                          "partitioner": plan["partitioner"], "parts": len(plan["parts"]),
                          "ms": float(times.sum())})
-        if (c + 1) % 100 == 0:
-            log(f"[collect] {c + 1}/{num} circuits, {len(X)} parts, {time.time() - t0:.0f} s")
-    for s in states.values():
+        if (c + 1) % 50 == 0:
+            log(f"[collect] {c + 1}/{num} circuits (n={n}), {len(X)} parts, {time.time() - t0:.0f} s")
+    if s is not None:
         s.close()
     return np.array(X, np.float32), np.array(y, np.float32), meta
 
@@ -107,9 +117,27 @@ def main(argv=None) -> int:
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--out", default="gpurun_out")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--stage", choices=("all", "collect", "fit"), default="all",
+                    help="collect: time circuits and save the samples to --data; fit: train and evaluate "
+                         "on the samples in --data (several files, comma-separated); all: both")
+    ap.add_argument("--data", default="")
+    ap.add_argument("--max-seconds", type=float, default=0.0, help="time budget of the collection")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
-    X, y, meta = collect(args.circuits, args.seed)
+    if args.stage == "fit":
+        parts = [np.load(p) for p in args.data.split(",") if p]
+        X = np.concatenate([d["X"] for d in parts])
+        y = np.concatenate([d["y"] for d in parts])
+        meta = []
+        args.circuits = int(sum(int(d["circuits"]) for d in parts))
+    else:
+        X, y, meta = collect(args.circuits, args.seed, max_seconds=args.max_seconds)
+        args.circuits = len(meta) // 2
+        if args.data:
+            np.savez(args.data, X=X, y=y, circuits=args.circuits)
+            log(f"[collect] saved {len(X)} samples to {args.data}")
+        if args.stage == "collect":
+            return 0
     log(f"[train] {len(X)} samples")
     rng = np.random.default_rng(1)
     idx = rng.permutation(len(X))
