@@ -53,7 +53,42 @@ def make_parts(gates, mode: str, k: int):
     return greedy_parts(gates, k)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(kernel: str, n: int, mode: str, k: int):
+    """HBM bytes per launch of the dominant kernel for this workload, from
+    the committed PMC summary (FETCH_SIZE + WRITE_SIZE per dispatch, each
+    scaled by the counter's reading on a copy kernel of known size), or None
+    when that workload was not profiled."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    row = table.get(f"{kernel}|{mode}|n{n}|k{k}")
+    return None if row is None else row.get("bytes_per_launch")
+
+
+def exchange_permutation(n: int, m: int = 3):
+    """The qubit permutation of one exchange step (svsim-mpi.hpp:139-149, a
+    stable partition of the slots): qubits 9..9+m-1 move to the top m slots
+    and every other qubit keeps its relative order, so the top m qubits
+    move down by m. Slots below 9 are untouched, which keeps 512-amplitude
+    runs whole."""
+    mid = list(range(9, 9 + m))
+    new_slot = [0] * n
+    # qubits in ``mid`` move to the top slots, the others fill the rest in order
+    rest = [q for q in range(n) if q not in mid]
+    for slot, q in enumerate(rest + mid):
+        new_slot[q] = slot
+    return new_slot
+
+
 def run_path(s: capi.State, mode: str, gates, parts) -> None:
+    if mode == "permute":
+        s.permute_qubits(parts)
+        return
     s.reset()
     if mode == "pergate":
         s.apply_gates(gates)
@@ -69,6 +104,17 @@ def run_path(s: capi.State, mode: str, gates, parts) -> None:
 def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0) -> dict:
     """Infidelity and max |d amp| of the chosen path against the FP64 oracle."""
     from oracle.oracle import Oracle
+    if mode == "permute":
+        rng = np.random.default_rng(seed)
+        a = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+        perm = exchange_permutation(n, k)
+        want = Oracle().permute(a, perm)
+        with capi.State(n) as s:
+            s.write(a)
+            run_path(s, mode, [], perm)
+            got = s.read()
+        return {"n": n, "infidelity": None, "max_abs_err": float(np.abs(got - want).max()),
+                "bit_exact": bool(np.array_equal(got, want))}
     gates = random_layered(n, depth, seed=seed)
     parts = make_parts(gates, mode, k)
     want = np.zeros(2 ** n, complex)
@@ -119,10 +165,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=33)
+    ap.add_argument("--n", type=int, default=0, help="qubits (default 33; 32 for --mode permute, which "
+                                                      "needs a second state buffer)")
     ap.add_argument("--depth", type=int, default=40)
-    ap.add_argument("--mode", default="part", choices=["part", "fused64", "fused32", "pergate"])
-    ap.add_argument("--k", type=int, default=0, help="part width (default: per mode)")
+    ap.add_argument("--mode", default="part", choices=["part", "fused64", "fused32", "pergate", "permute"])
+    ap.add_argument("--k", type=int, default=0, help="part width (default: per mode; permute: qubits exchanged)")
     ap.add_argument("--pergate-layers", type=int, default=1, help="layers of the per-gate timing sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
@@ -131,10 +178,13 @@ def main() -> int:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU layout is described "
               f"in DESIGN.md and not run here.", file=sys.stderr)
         return 2
-    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "pergate": 1}[args.mode]
-    n, depth = args.n, args.depth
+    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "pergate": 1, "permute": 3}[args.mode]
+    n, depth = args.n or (32 if args.mode == "permute" else 33), args.depth
     gates = random_layered(n, depth, seed=0)
-    parts = make_parts(gates, args.mode, k) if args.mode != "pergate" else []
+    if args.mode == "permute":
+        parts = exchange_permutation(n, k)
+    else:
+        parts = make_parts(gates, args.mode, k) if args.mode != "pergate" else []
     S = 16
     total_upd = sum(gate_amplitudes(g, n) for g in gates)
     log(f"[bench] n={n} depth={depth} gates={len(gates)} mode={args.mode} k={k} parts={len(parts)}")
@@ -163,33 +213,39 @@ def main() -> int:
         passes = launches.shape[1]
         log(f"[bench] TTS {tts:.3f} s, {passes} passes/step, mean pass {kern_ms:.3f} ms")
 
-        # per-gate sample: first layers of the same circuit
-        per_layer = n + n // 2
-        sample = gates[: per_layer * args.pergate_layers]
-        s.reset()
-        s.synchronize()
-        s.set_timing(True)
-        s.kernel_times()
-        s.apply_gates(sample)
-        s.synchronize()
-        gt = s.kernel_times()
-        s.set_timing(False)
-        g_bytes = np.array([2 * S * gate_amplitudes(g, n) for g in sample], float)
-        g_ms = gt[: len(sample)]
-        dense = np.array([g.name == "U3" for g in sample])
-        pergate_tts = float(sum((2 * S * gate_amplitudes(g, n)) for g in gates) /
-                            (g_bytes.sum() / (g_ms.sum() * 1e-3)))
-        pergate_gbs_dense = float(g_bytes[dense].sum() / (g_ms[dense].sum() * 1e-3) / 1e9)
-        pergate_gbs = float(g_bytes.sum() / (g_ms.sum() * 1e-3) / 1e9)
-        log(f"[bench] per-gate sample {len(sample)} gates: {pergate_gbs:.0f} GB/s overall, "
-            f"{pergate_gbs_dense:.0f} GB/s dense U3; extrapolated per-gate TTS {pergate_tts:.2f} s")
+        per_gate = None
+        if args.mode != "permute":
+            # per-gate sample: first layers of the same circuit
+            per_layer = n + n // 2
+            sample = gates[: per_layer * args.pergate_layers]
+            s.reset()
+            s.synchronize()
+            s.set_timing(True)
+            s.kernel_times()
+            s.apply_gates(sample)
+            s.synchronize()
+            gt = s.kernel_times()
+            s.set_timing(False)
+            g_bytes = np.array([2 * S * gate_amplitudes(g, n) for g in sample], float)
+            g_ms = gt[: len(sample)]
+            dense = np.array([g.name == "U3" for g in sample])
+            pergate_tts = float(sum((2 * S * gate_amplitudes(g, n)) for g in gates) /
+                                (g_bytes.sum() / (g_ms.sum() * 1e-3)))
+            pergate_gbs_dense = float(g_bytes[dense].sum() / (g_ms[dense].sum() * 1e-3) / 1e9)
+            pergate_gbs = float(g_bytes.sum() / (g_ms.sum() * 1e-3) / 1e9)
+            log(f"[bench] per-gate sample {len(sample)} gates: {pergate_gbs:.0f} GB/s overall, "
+                f"{pergate_gbs_dense:.0f} GB/s dense U3; extrapolated per-gate TTS {pergate_tts:.2f} s")
+            per_gate = {"sample_gates": len(sample), "gbs": pergate_gbs, "gbs_dense_u3": pergate_gbs_dense,
+                        "tts_extrapolated_s": pergate_tts, "passes": len(gates),
+                        "speedup_of_mode": pergate_tts / tts}
 
-    if args.mode in ("part", "pergate"):
+    if args.mode in ("part", "pergate", "permute"):
         unit_bytes = 2 * S * 2 ** n
         achieved = unit_bytes / (kern_ms * 1e-3) / 1e9
+        kernel = {"part": "k_part_reg", "pergate": "k_gate", "permute": "k_permute"}[args.mode]
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_part" if args.mode == "part" else "k_gate",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, n, args.mode, k),
+                "kernel": kernel,
                 "per_launch": f"{unit_bytes} B (2 x 16 B x 2^{n}: read + write of the state) / {kern_ms:.3f} ms"}
     else:
         kk = max(4, min(7, k))
@@ -197,15 +253,16 @@ def main() -> int:
         peak = FP64_MFMA_PEAK if args.mode == "fused64" else FP32_MFMA_PEAK
         achieved = flops / (kern_ms * 1e-3)
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "kernel": "k_fused",
+                "frac": achieved / peak, "traffic": measured_traffic("k_fused", n, args.mode, k),
+                "kernel": "k_fused",
                 "per_launch": f"8 x 2^k x 2^{n} algorithmic flops (k={kk}) / {kern_ms:.3f} ms",
                 "hbm_gbs": 2 * S * 2 ** n / (kern_ms * 1e-3) / 1e9}
 
     verify = None if args.no_verify else verify_small(args.mode, k, depth)
     if verify:
-        log(f"[bench] verify vs FP64 oracle at n={verify['n']}: infidelity {verify['infidelity']:.2e}, "
+        log(f"[bench] verify vs FP64 oracle at n={verify['n']}: infidelity {verify['infidelity']}, "
             f"max|d| {verify['max_abs_err']:.2e}")
-    cpu = None if args.no_cpu else cpu_baseline(n, depth, k)
+    cpu = None if args.no_cpu or args.mode == "permute" else cpu_baseline(n, depth, k)
     if cpu:
         log(f"[bench] cpu baseline: {cpu}")
 
@@ -214,14 +271,16 @@ def main() -> int:
         "ms_per_step": tts * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64" if args.mode != "fused32" else "f32split",
         "data": "synthetic: seeded random layered circuit (U3 angles ~ U[0,2pi), CZ on random matchings), |0..0> input",
-        "config": {"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, complex128 state "
-                               f"({2 ** n * 16 / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
-                   "mode": args.mode, "k": k, "passes": passes, "partitioner": "greedy acyclic"},
+        "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, complex128 state "
+                                f"({2 ** n * 16 / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
+                    "mode": args.mode, "k": k, "passes": passes, "partitioner": "greedy acyclic"}
+                   if args.mode != "permute" else
+                   {"workload": f"exchange permutation of a {n}-qubit complex128 state ({2 ** n * 16 / 1e9:.1f} GB): "
+                                f"qubits 9..{8 + k} to the top slots", "n": n, "mode": "permute", "k": k,
+                    "passes": passes}),
         "hbm_gbs": 2 * S * 2 ** n / (kern_ms * 1e-3) / 1e9,
         "gate_amp_updates_per_s": total_upd / tts,
-        "per_gate": {"sample_gates": len(sample), "gbs": pergate_gbs, "gbs_dense_u3": pergate_gbs_dense,
-                     "tts_extrapolated_s": pergate_tts, "passes": len(gates),
-                     "speedup_of_mode": pergate_tts / tts},
+        "per_gate": per_gate,
         "roofline": roof,
         "verify": verify,
         "cpu_baseline": cpu,

@@ -96,6 +96,8 @@ SIGNATURES = {
     "hsv_max_abs_diff": (ctypes.c_int, [_P, _P, _D]),
     "hsv_probability": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
     "hsv_measure": (ctypes.c_int, [_P, _I, ctypes.c_int, _D, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
+    "hsv_project": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
+    "hsv_expect_pauli": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
     "hsv_gate_matrix": (ctypes.c_int, [_G, _D, _I]),
     "hsv_fused_unitary": (ctypes.c_int, [ctypes.c_int, _G, ctypes.c_int, _I, _I, _D, _D]),
     "hsv_synchronize": (ctypes.c_int, [_P]),
@@ -152,6 +154,24 @@ def _check(rc: int) -> None:
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(_D)
+
+
+def pauli_masks(spec: str):
+    """``"X0 Z3 Y5"`` -> (x_mask, z_mask): X sets the x bit, Z the z bit, Y
+    both; ``I`` factors and an empty string are the identity."""
+    xm = zm = 0
+    for tok in spec.split():
+        p, q = tok[0].upper(), int(tok[1:])
+        if p not in "IXYZ" or q < 0:
+            raise ValueError(f"bad Pauli factor {tok!r}")
+        bit = 1 << q
+        if (xm | zm) & bit:
+            raise ValueError(f"qubit {q} appears twice in {spec!r}")
+        if p in "XY":
+            xm |= bit
+        if p in "ZY":
+            zm |= bit
+    return xm, zm
 
 
 def gate_matrix(g) -> np.ndarray:
@@ -272,6 +292,29 @@ class State:
         _check(lib().hsv_measure(self._h, q, len(qubits), u.ctypes.data_as(_D), len(u),
                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return out
+
+    def project(self, qubits: Sequence[int], values: Sequence[int] = ()) -> float:
+        """Collapses the qubits to the values (0 where not given, as the
+        reference's default), renormalised; returns the outcome's
+        probability."""
+        values = list(values) + [0] * (len(qubits) - len(values))
+        mask = val = 0
+        for q, v in zip(qubits, values):
+            if v not in (0, 1) or (mask >> q) & 1:
+                raise ValueError("values must be 0/1 and qubits unique")
+            mask |= 1 << q
+            val |= v << q
+        p = ctypes.c_double()
+        _check(lib().hsv_project(self._h, mask, val, ctypes.byref(p)))
+        return p.value
+
+    def expectation(self, pauli) -> float:
+        """<psi|P|psi> for a Pauli string given as ``"X0 Z3 Y5"`` (factor
+        letter + qubit, identity elsewhere) or as an (x_mask, z_mask) pair."""
+        xm, zm = pauli_masks(pauli) if isinstance(pauli, str) else pauli
+        v = (ctypes.c_double * 2)()
+        _check(lib().hsv_expect_pauli(self._h, int(xm), int(zm), v))
+        return v[0]
 
     def tune(self, key: str, value: int) -> None:
         _check(lib().hsv_tune(self._h, key.encode(), int(value)))

@@ -682,6 +682,66 @@ int hsv_probability(hsv_state* s, int qubit, int value, double* out) {
   return HSV_OK;
 }
 
+int hsv_project(hsv_state* s, uint64_t qubit_mask, uint64_t values, double* p_out) {
+  if (!s) return fail(HSV_ERR_ARG, "null handle");
+  if (qubit_mask >> s->n) return fail(HSV_ERR_ARG, "qubit mask names a qubit beyond the state");
+  if (values & ~qubit_mask) return fail(HSV_ERR_ARG, "values has bits outside qubit_mask");
+  const int blocks = kRedBlocks;
+  if (s->dtype == HSV_C128)
+    hipLaunchKernelGGL((k_mass<double>), blocks, kRedBlock, 0, s->stream, amp_as<double>(s), s->len, qubit_mask,
+                       values, s->partial);
+  else
+    hipLaunchKernelGGL((k_mass<float>), blocks, kRedBlock, 0, s->stream, amp_as<float>(s), s->len, qubit_mask,
+                       values, s->partial);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(s->host_partial.data(), s->partial, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost,
+                        s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  double p = 0;
+  for (int i = 0; i < blocks; ++i) p += s->host_partial[2 * i];
+  if (p_out) *p_out = p;
+  if (!(p > 0.0)) return fail(HSV_ERR_ARG, "the outcome has probability 0");
+  const double scale = 1.0 / std::sqrt(p);
+  const int grid = grid_for(s->len, 256, 1u << 16);
+  if (s->dtype == HSV_C128)
+    hipLaunchKernelGGL((k_project<double>), grid, 256, 0, s->stream, amp_as<double>(s), s->len, qubit_mask, values,
+                       scale);
+  else
+    hipLaunchKernelGGL((k_project<float>), grid, 256, 0, s->stream, amp_as<float>(s), s->len, qubit_mask, values,
+                       scale);
+  HIPCHK(hipGetLastError());
+  return HSV_OK;
+}
+
+int hsv_expect_pauli(hsv_state* s, uint64_t x_mask, uint64_t z_mask, double* out2) {
+  if (!s || !out2) return fail(HSV_ERR_ARG, "null argument");
+  if ((x_mask | z_mask) >> s->n) return fail(HSV_ERR_ARG, "Pauli mask names a qubit beyond the state");
+  const int blocks = kRedBlocks;
+  if (s->dtype == HSV_C128)
+    hipLaunchKernelGGL((k_pauli<double>), blocks, kRedBlock, 0, s->stream, amp_as<double>(s), s->len, x_mask,
+                       z_mask, s->partial);
+  else
+    hipLaunchKernelGGL((k_pauli<float>), blocks, kRedBlock, 0, s->stream, amp_as<float>(s), s->len, x_mask,
+                       z_mask, s->partial);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(s->host_partial.data(), s->partial, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost,
+                        s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  double re = 0, im = 0;
+  for (int i = 0; i < blocks; ++i) {
+    re += s->host_partial[2 * i];
+    im += s->host_partial[2 * i + 1];
+  }
+  // multiply by i^ny, ny = number of Y factors
+  switch (__builtin_popcountll(x_mask & z_mask) & 3) {
+    case 0: out2[0] = re; out2[1] = im; break;
+    case 1: out2[0] = -im; out2[1] = re; break;
+    case 2: out2[0] = -re; out2[1] = -im; break;
+    default: out2[0] = im; out2[1] = -re; break;
+  }
+  return HSV_OK;
+}
+
 int hsv_measure(hsv_state* s, const int* qubits, int num_qubits, const double* uniforms, int num_samples,
                 int32_t* out) {
   if (!s || num_qubits < 0 || num_samples < 0 || (num_qubits && !qubits) || (num_samples && (!uniforms || !out)))

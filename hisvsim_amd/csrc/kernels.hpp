@@ -345,6 +345,75 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce(const typename VecT<R>::ty
   }
 }
 
+// <psi|P|psi> for a Pauli string: P|j> = i^ny (-1)^popc(j & z) |j ^ x>, so the
+// sum is i^ny * sum_j conj(a[j ^ x]) a[j] (-1)^popc(j & z); the host applies
+// i^ny. Same fixed grid and partials layout as k_reduce.
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void k_pauli(const typename VecT<R>::type* __restrict__ a, uint64_t len,
+                                                     uint64_t xmask, uint64_t zmask, double* __restrict__ partial) {
+  double s0 = 0.0, s1 = 0.0;
+  const uint64_t stride = uint64_t(gridDim.x) * kRedBlock;
+  for (uint64_t j = uint64_t(blockIdx.x) * kRedBlock + threadIdx.x; j < len; j += stride) {
+    const double x = a[j].x, y = a[j].y;
+    const uint64_t k = j ^ xmask;
+    const double u = a[k].x, v = a[k].y;
+    const double sg = (__popcll(j & zmask) & 1) ? -1.0 : 1.0;
+    s0 += sg * (u * x + v * y);   // Re(conj(a_k) a_j)
+    s1 += sg * (u * y - v * x);   // Im(conj(a_k) a_j)
+  }
+  __shared__ double r0[kRedBlock], r1[kRedBlock];
+  r0[threadIdx.x] = s0;
+  r1[threadIdx.x] = s1;
+  __syncthreads();
+  for (int h = kRedBlock / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      r0[threadIdx.x] += r0[threadIdx.x + h];
+      r1[threadIdx.x] += r1[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = r0[0];
+    partial[2 * blockIdx.x + 1] = r1[0];
+  }
+}
+
+// Probability mass of the amplitudes with (i & mask) == value.
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void k_mass(const typename VecT<R>::type* __restrict__ a, uint64_t len,
+                                                    uint64_t mask, uint64_t value, double* __restrict__ partial) {
+  double s0 = 0.0;
+  const uint64_t stride = uint64_t(gridDim.x) * kRedBlock;
+  for (uint64_t i = uint64_t(blockIdx.x) * kRedBlock + threadIdx.x; i < len; i += stride) {
+    if ((i & mask) != value) continue;
+    const double x = a[i].x, y = a[i].y;
+    s0 += x * x + y * y;
+  }
+  __shared__ double r0[kRedBlock];
+  r0[threadIdx.x] = s0;
+  __syncthreads();
+  for (int h = kRedBlock / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) r0[threadIdx.x] += r0[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = r0[0];
+    partial[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+// Projection onto (i & mask) == value with renormalisation by `scale`.
+template <typename R>
+__global__ __launch_bounds__(256) void k_project(typename VecT<R>::type* __restrict__ a, uint64_t len, uint64_t mask,
+                                                 uint64_t value, double scale) {
+  using V = typename VecT<R>::type;
+  const uint64_t stride = uint64_t(gridDim.x) * 256;
+  for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < len; i += stride) {
+    const V v = a[i];
+    a[i] = (i & mask) == value ? cmk<V, R>(R(v.x * scale), R(v.y * scale)) : cmk<V, R>(R(0), R(0));
+  }
+}
+
 // ---- fill |0...0> -----------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_zero_state(typename VecT<R>::type* __restrict__ a, uint64_t len) {

@@ -215,6 +215,29 @@ struct StateVector {
     check(hsv_inner(h, o.h, v), "inner");
     return cd{v[0], v[1]};
   }
+  // state_vector.hpp:829-895 (values default to 0), renormalised by
+  // 1/sqrt(p); the qubits stay in the state. Returns p.
+  double project(const std::vector<int>& qubits, std::vector<int> values) {
+    values.resize(qubits.size(), 0);
+    uint64_t mask = 0, val = 0;
+    for (size_t i = 0; i < qubits.size(); ++i) {
+      if (qubits[i] < 0 || qubits[i] >= n || (mask >> qubits[i] & 1) || (values[i] != 0 && values[i] != 1))
+        throw py::value_error("project: qubits must be unique state qubits and values 0 or 1");
+      mask |= 1ull << qubits[i];
+      val |= uint64_t(values[i]) << qubits[i];
+    }
+    double p = 0;
+    py::gil_scoped_release nogil;
+    check(hsv_project(h, mask, val, &p), "project");
+    return p;
+  }
+  // <psi|P|psi>; X on x_mask only, Z on z_mask only, Y on both.
+  double expectation(uint64_t x_mask, uint64_t z_mask) const {
+    double v[2] = {0, 0};
+    py::gil_scoped_release nogil;
+    check(hsv_expect_pauli(h, x_mask, z_mask, v), "expectation");
+    return v[0];
+  }
   // state_vector.hpp:1003-1010: every amplitude within 1e-10 (types.hpp:10).
   bool state_equal(const StateVector& o) const {
     if (o.n != n) return false;
@@ -306,6 +329,8 @@ PYBIND11_MODULE(svsimulator_py_wrapper, m) {
       .def("set_amplitudes", &StateVector::set_amplitudes, py::arg("amps"), py::arg("offset") = 0)
       .def("norm2", &StateVector::norm2)
       .def("probability", &StateVector::probability, py::arg("qubit"), py::arg("value") = 0)
+      .def("project", &StateVector::project, py::arg("qubits"), py::arg("values") = std::vector<int>{})
+      .def("expectation", &StateVector::expectation, py::arg("x_mask"), py::arg("z_mask"))
       .def("max_abs_diff", &StateVector::max_abs_diff)
       .def("inner", &StateVector::inner)
       .def("reset", [](StateVector& s) { check(hsv_reset(s.h), "reset"); })

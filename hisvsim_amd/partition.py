@@ -19,9 +19,14 @@ valid topological order of the gate DAG):
 * ``layered_parts`` -- for circuits made of single-qubit layers and two-qubit
   matchings (the random layered benchmark): groups each matching's pairs so a
   part holds whole pairs plus the single-qubit gates that follow them.
+* ``dfs_parts`` -- natural-order parts over a depth-first topological order
+  of the gate DAG (the method of the shipped ``*_part_dfs`` files, with this
+  package's own DFS).
+* ``merge_parts`` -- refinement: joins neighbouring parts whose qubits fit.
 """
 from __future__ import annotations
 
+import heapq
 from collections import OrderedDict
 from typing import Dict, List, Sequence
 
@@ -32,15 +37,21 @@ def gate_qubits(g: Gate) -> set:
     return set(g.controls) | set(g.targets)
 
 
-def nat_parts(gates: Sequence[Gate], k: int) -> List[List[Gate]]:
+def nat_parts(gates: Sequence[Gate], k: int, always: Sequence[int] = ()) -> List[List[Gate]]:
+    """``always`` as in ``greedy_parts``: qubits counted in every part."""
+    base = set(always)
+    if len(base) > k:
+        raise ValueError("more always-included qubits than the part width")
     parts: List[List[Gate]] = []
     cur: List[Gate] = []
-    qs: set = set()
+    qs: set = set(base)
     for g in gates:
         gq = gate_qubits(g)
+        if len(base | gq) > k:
+            raise ValueError(f"gate {g} does not fit a part of {k} qubits that already holds {sorted(base)}")
         if cur and len(qs | gq) > k:
             parts.append(cur)
-            cur, qs = [], set()
+            cur, qs = [], set(base)
         cur.append(g)
         qs |= gq
     if cur:
@@ -171,6 +182,81 @@ def layered_parts(gates: Sequence[Gate], n: int, k: int) -> List[List[Gate]]:
             if part:
                 parts.append(part)
     return parts
+
+
+def gate_dag(gates: Sequence[Gate]) -> List[List[int]]:
+    """Successor lists of the gate DAG: an edge from each gate to the next
+    gate on each of its qubits (the edges of the reference's DAG files,
+    qasm_assembler_standalone.py:3688-3764, without input/exit nodes)."""
+    succ: List[List[int]] = [[] for _ in gates]
+    last: Dict[int, int] = {}
+    for i, g in enumerate(gates):
+        for q in sorted(gate_qubits(g)):
+            if q in last and i not in succ[last[q]]:
+                succ[last[q]].append(i)
+            last[q] = i
+    return succ
+
+
+def dfs_order(gates: Sequence[Gate]) -> List[int]:
+    """A topological order of the gate DAG that follows one dependency chain
+    as deep as it can before backtracking (depth-first): among the ready
+    gates, take a successor of the gate just placed when one is ready, else
+    the earliest ready gate. The reference's shipped ``*_part_dfs`` files
+    come from a DFS ordering of the same DAG by an external tool; this is
+    this package's own."""
+    succ = gate_dag(gates)
+    indeg = [0] * len(gates)
+    for s in succ:
+        for j in s:
+            indeg[j] += 1
+    ready = sorted(i for i in range(len(gates)) if indeg[i] == 0)
+    stack: List[int] = []
+    order: List[int] = []
+    heapq.heapify(ready)
+    placed = [False] * len(gates)
+    while len(order) < len(gates):
+        nxt = None
+        while stack and nxt is None:
+            cand = [j for j in succ[stack[-1]] if indeg[j] == 0 and not placed[j]]
+            if cand:
+                nxt = min(cand)
+            else:
+                stack.pop()
+        if nxt is None:
+            while placed[ready[0]]:
+                heapq.heappop(ready)
+            nxt = heapq.heappop(ready)
+        placed[nxt] = True
+        order.append(nxt)
+        stack.append(nxt)
+        for j in succ[nxt]:
+            indeg[j] -= 1
+            if indeg[j] == 0:
+                heapq.heappush(ready, j)
+    return order
+
+
+def dfs_parts(gates: Sequence[Gate], k: int, always: Sequence[int] = ()) -> List[List[Gate]]:
+    """Natural-order parts over the depth-first topological order."""
+    return nat_parts([gates[i] for i in dfs_order(gates)], k, always)
+
+
+def merge_parts(parts: Sequence[Sequence[Gate]], k: int) -> List[List[Gate]]:
+    """Refinement: joins each part with the next while their qubit union
+    stays within k. Joining neighbours in the sequence keeps every qubit's
+    gate order, so any valid sequence stays valid."""
+    out: List[List[Gate]] = []
+    qs: set = set()
+    for p in parts:
+        pq = set().union(*[gate_qubits(g) for g in p]) if p else set()
+        if out and len(qs | pq) <= k:
+            out[-1] = out[-1] + list(p)
+            qs |= pq
+        else:
+            out.append(list(p))
+            qs = set(pq)
+    return out
 
 
 def check_parts(gates: Sequence[Gate], parts: Sequence[Sequence[Gate]], k: int | None = None) -> None:

@@ -120,6 +120,19 @@ int hsv_probability(hsv_state* s, int qubit, int value, double* out);
  * out[i*num_qubits + j] = bit of qubits[j] in sample i. */
 int hsv_measure(hsv_state* s, const int* qubits, int num_qubits,
                 const double* uniforms, int num_samples, int32_t* out);
+/* Replaces StateVector::project (state_vector.hpp:829-895): keeps the
+ * amplitudes whose bits under qubit_mask equal `values` (bit q = qubit q),
+ * zeroes the others and renormalises by 1/sqrt(p), p = the outcome's
+ * probability before the projection, written to *p_out. Deviations: the
+ * reference divides by p (:880-882) and removes the projected qubits from the
+ * state; here they stay, fixed at their values. p = 0 is an error
+ * (HSV_ERR_ARG) and leaves the state unchanged. */
+int hsv_project(hsv_state* s, uint64_t qubit_mask, uint64_t values, double* p_out);
+/* <psi|P|psi> for the Pauli string P with X on the qubits of x_mask only, Z
+ * on z_mask only and Y on both (bit q = qubit q), FP64 accumulation. The
+ * reference has no expectation read-out; its tests measure instead. out2 =
+ * (re, im); im is rounding noise for a Hermitian P. */
+int hsv_expect_pauli(hsv_state* s, uint64_t x_mask, uint64_t z_mask, double* out2);
 
 /* Host-only introspection (no device needed; not in the reference).
  * hsv_gate_matrix: the 2x2 or 4x4 matrix a gate is lowered to, row-major

@@ -79,6 +79,26 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def project_kept(state: np.ndarray, qubits: Sequence[int], values: Sequence[int]) -> np.ndarray:
+    """The selection of StateVector::project (state_vector.hpp:829-895,
+    loop :857-863): the amplitudes whose projected bits equal the values,
+    in index order over the remaining qubits; not renormalised."""
+    idx = np.arange(state.size)
+    keep = np.ones(state.size, bool)
+    for q, v in zip(qubits, values):
+        keep &= ((idx >> q) & 1) == v
+    return state[keep]
+
+
+def pauli_expectation(state: np.ndarray, x_mask: int, z_mask: int) -> complex:
+    """<psi|P|psi> with P|j> = i^ny (-1)^popcount(j & z) |j ^ x> (X on x
+    only, Z on z only, Y on both; ny = popcount(x & z))."""
+    idx = np.arange(state.size)
+    sign = 1 - 2 * (np.array([bin(int(v)).count("1") for v in (idx & z_mask)]) & 1)
+    ny = bin(x_mask & z_mask).count("1")
+    return complex((1j ** ny) * np.sum(np.conj(state[idx ^ x_mask]) * sign * state))
+
+
 def _check_state(state: np.ndarray) -> int:
     if state.dtype != np.complex128 or not state.flags.c_contiguous:
         raise TypeError("state must be a C-contiguous complex128 array")
@@ -201,6 +221,36 @@ class RefSim:
             self.lib.ref_destroy(sv)
         if rc:
             raise ValueError(f"reference harness refused the call (code {rc})")
+
+    def probability(self, state: np.ndarray, qubit: int, value: int = 0) -> float:
+        """StateVector::probability (state_vector.hpp:897-923)."""
+        n = _check_state(state)
+        L = self.lib
+        L.ref_probability.restype = ctypes.c_double
+        L.ref_probability.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        sv = L.ref_create(n)
+        try:
+            L.ref_write(sv, _ptr(state), n)
+            return float(L.ref_probability(sv, qubit, value))
+        finally:
+            L.ref_destroy(sv)
+
+    def project(self, state: np.ndarray, qubits: Sequence[int], values: Sequence[int]) -> np.ndarray:
+        """StateVector::project (state_vector.hpp:829-895): the 2^(n-k)
+        amplitudes over the remaining qubits, divided by p."""
+        n = _check_state(state)
+        L = self.lib
+        L.ref_project.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        k = len(qubits)
+        out = np.zeros(2 ** (n - k), complex)
+        sv = L.ref_create(n)
+        try:
+            L.ref_write(sv, _ptr(state), n)
+            L.ref_project(sv, (ctypes.c_int * k)(*qubits), (ctypes.c_int * k)(*values), k, _ptr(out), n)
+        finally:
+            L.ref_destroy(sv)
+        return out
 
     def apply_gates(self, state: np.ndarray, gates: Sequence[Gate]) -> None:
         descs = [to_desc(g) for g in gates]

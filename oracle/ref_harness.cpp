@@ -122,6 +122,22 @@ int ref_apply_gate(void* p, const GateDesc* d) {
   return 0;
 }
 
+// StateVector::probability (state_vector.hpp:897-923).
+double ref_probability(void* p, int qubit, int value) {
+  return static_cast<StateVector*>(p)->probability(static_cast<Qubit>(qubit), value);
+}
+
+// StateVector::project (state_vector.hpp:829-895): the state shrinks to the
+// 2^(n - count) amplitudes of the remaining qubits, written to out.
+void ref_project(void* p, const int* qubits, const int* values, int count, double* out, int n) {
+  StateVector* sv = static_cast<StateVector*>(p);
+  std::vector<Qubit> q(qubits, qubits + count);
+  std::vector<int> v(values, values + count);
+  sv->project(q, v, true);
+  const Type* s = sv->state();
+  for (uint64_t i = 0; i < (1ull << (n - count)); ++i) { out[2 * i] = s[i].real(); out[2 * i + 1] = s[i].imag(); }
+}
+
 int ref_run_part(void* p, const GateDesc* ds, int ngates, int opt_slots, int n) {
   uint64_t part = 0;
   for (int i = 0; i < ngates; ++i) {

@@ -122,13 +122,44 @@ def circuits_fixture(ref):
     return out
 
 
+PROJECT_CASES = [((0,), (0,)), ((3,), (1,)), ((7,), (1,)), ((1, 5), (1, 0)), ((6, 2, 4), (0, 1, 1)), ((2,), ())]
+
+
+def readouts_fixture(ref):
+    """StateVector::probability (every qubit, both values) and ::project
+    (single and several qubits; an omitted value defaults to 0) on seeded
+    random states, n=8."""
+    n = 8
+    out = {"n": n}
+    for i, (qs, vs) in enumerate(PROJECT_CASES):
+        a = rand_state(n, 500 + i)
+        out[f"proj{i}_in"] = a
+        out[f"proj{i}_qubits"] = np.array(qs)
+        out[f"proj{i}_values"] = np.array(list(vs) + [0] * (len(qs) - len(vs)))
+        out[f"proj{i}_out"] = ref.project(a, qs, list(vs) + [0] * (len(qs) - len(vs)))
+    # probability accumulates into one shared double inside the OpenMP loop
+    # (state_vector.hpp:908-913), a data race with several threads: take it
+    # with one thread, where it is the plain marginal
+    ref.set_threads(1)
+    a = rand_state(n, 600)
+    out["prob_in"] = a
+    out["prob"] = np.array([[ref.probability(a, q, v) for v in (0, 1)] for q in range(n)])
+    ref.set_threads(8)
+    return out
+
+
+FIXTURES = {"gates": ("ref_gates.npz", gates_fixture), "circuits": ("ref_circuits.npz", circuits_fixture),
+            "readouts": ("ref_readouts.npz", readouts_fixture)}
+
+
 def main():
+    """``python make_golden.py [gates] [circuits] [readouts]`` (default: all)."""
     ref = RefSim()
     ref.set_threads(8)
-    np.savez_compressed(os.path.join(HERE, "ref_gates.npz"), **gates_fixture(ref))
-    np.savez_compressed(os.path.join(HERE, "ref_circuits.npz"), **circuits_fixture(ref))
-    for f in ("ref_gates.npz", "ref_circuits.npz"):
-        print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
+    for key in (sys.argv[1:] or list(FIXTURES)):
+        fname, fn = FIXTURES[key]
+        np.savez_compressed(os.path.join(HERE, fname), **fn(ref))
+        print(fname, os.path.getsize(os.path.join(HERE, fname)), "bytes")
 
 
 if __name__ == "__main__":
