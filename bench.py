@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 
 from hisvsim_amd import capi  # noqa: E402
 from hisvsim_amd.circuit import gate_amplitudes, random_layered  # noqa: E402
-from hisvsim_amd.partition import greedy_parts  # noqa: E402
+from hisvsim_amd.partition import dfs_parts, greedy_parts  # noqa: E402
 
 METRIC = "33q circuit time-to-solution + HBM GB/s; fused-MFMA vs per-gate at ≤1e-6 infid"
 HBM_PEAK_GBS = 8000.0
@@ -44,13 +44,17 @@ def log(msg: str) -> None:
 LOW_SLOTS = 5  # qubits the device adds to every LDS-staged part (libhsv default)
 
 
-def make_parts(gates, mode: str, k: int):
-    """Greedy acyclic parts. For the LDS/register part kernel, k counts the
-    low qubits the device adds to every part anyway, so the partitioner
-    starts each part with them."""
-    if mode == "part":
-        return greedy_parts(gates, k, always=range(LOW_SLOTS))
-    return greedy_parts(gates, k)
+def make_parts(gates, mode: str, k: int, partitioner: str = "best"):
+    """Acyclic parts: DAG-aware greedy, depth-first order, or whichever of
+    the two gives fewer parts ("best"). For the register part kernel, k
+    counts the low qubits the device adds to every part anyway, so the
+    partitioner starts each part with them."""
+    always = range(LOW_SLOTS) if mode == "part" else ()
+    cands = {"greedy": lambda: greedy_parts(gates, k, always=always),
+             "dfs": lambda: dfs_parts(gates, k, always=always)}
+    if partitioner != "best":
+        return cands[partitioner](), partitioner
+    return min(((f(), name) for name, f in cands.items()), key=lambda x: len(x[0]))
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
@@ -101,7 +105,7 @@ def run_path(s: capi.State, mode: str, gates, parts) -> None:
             s.apply_fused(p, fm)
 
 
-def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0) -> dict:
+def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0, partitioner: str = "best") -> dict:
     """Infidelity and max |d amp| of the chosen path against the FP64 oracle."""
     from oracle.oracle import Oracle
     if mode == "permute":
@@ -116,7 +120,7 @@ def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0) -> d
         return {"n": n, "infidelity": None, "max_abs_err": float(np.abs(got - want).max()),
                 "bit_exact": bool(np.array_equal(got, want))}
     gates = random_layered(n, depth, seed=seed)
-    parts = make_parts(gates, mode, k)
+    parts = make_parts(gates, mode, k, partitioner)[0] if mode != "pergate" else []
     want = np.zeros(2 ** n, complex)
     want[0] = 1
     o = Oracle()
@@ -170,6 +174,8 @@ def main() -> int:
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--mode", default="part", choices=["part", "fused64", "fused32", "pergate", "permute"])
     ap.add_argument("--k", type=int, default=0, help="part width (default: per mode; permute: qubits exchanged)")
+    ap.add_argument("--partitioner", default="best", choices=["best", "greedy", "dfs"],
+                    help="acyclic partitioner (best: the one with fewer parts)")
     ap.add_argument("--pergate-layers", type=int, default=1, help="layers of the per-gate timing sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
@@ -181,10 +187,13 @@ def main() -> int:
     k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "pergate": 1, "permute": 3}[args.mode]
     n, depth = args.n or (32 if args.mode == "permute" else 33), args.depth
     gates = random_layered(n, depth, seed=0)
+    partitioner_used = None
     if args.mode == "permute":
         parts = exchange_permutation(n, k)
+    elif args.mode == "pergate":
+        parts = []
     else:
-        parts = make_parts(gates, args.mode, k) if args.mode != "pergate" else []
+        parts, partitioner_used = make_parts(gates, args.mode, k, args.partitioner)
     S = 16
     total_upd = sum(gate_amplitudes(g, n) for g in gates)
     log(f"[bench] n={n} depth={depth} gates={len(gates)} mode={args.mode} k={k} parts={len(parts)}")
@@ -258,7 +267,7 @@ def main() -> int:
                 "per_launch": f"8 x 2^k x 2^{n} algorithmic flops (k={kk}) / {kern_ms:.3f} ms",
                 "hbm_gbs": 2 * S * 2 ** n / (kern_ms * 1e-3) / 1e9}
 
-    verify = None if args.no_verify else verify_small(args.mode, k, depth)
+    verify = None if args.no_verify else verify_small(args.mode, k, depth, partitioner=args.partitioner)
     if verify:
         log(f"[bench] verify vs FP64 oracle at n={verify['n']}: infidelity {verify['infidelity']}, "
             f"max|d| {verify['max_abs_err']:.2e}")
@@ -273,7 +282,7 @@ def main() -> int:
         "data": "synthetic: seeded random layered circuit (U3 angles ~ U[0,2pi), CZ on random matchings), |0..0> input",
         "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, complex128 state "
                                 f"({2 ** n * 16 / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
-                    "mode": args.mode, "k": k, "passes": passes, "partitioner": "greedy acyclic"}
+                    "mode": args.mode, "k": k, "passes": passes, "partitioner": partitioner_used}
                    if args.mode != "permute" else
                    {"workload": f"exchange permutation of a {n}-qubit complex128 state ({2 ** n * 16 / 1e9:.1f} GB): "
                                 f"qubits 9..{8 + k} to the top slots", "n": n, "mode": "permute", "k": k,

@@ -27,7 +27,7 @@ import numpy as np
 
 from . import capi
 from .circuit import Gate
-from .partition import greedy_parts, nat_parts
+from .partition import dfs_parts, greedy_parts, nat_parts
 
 MODES = ("part", "fused64")
 LOW_SLOTS = 5
@@ -139,20 +139,24 @@ class MLP:
 def candidate_specs() -> List[Tuple[str, int, str]]:
     """(mode, k, partitioner) candidates: the fused unitary at k = 5..7 and
     the part kernel at k = 8..13 (its parts already hold the 5 low qubits),
-    with the DAG-aware greedy or the natural-order partitioner (k = 5..10 for
-    natural order on the part kernel). k is the block width the model picks."""
+    with the DAG-aware greedy, the depth-first or the natural-order
+    partitioner (k = 5..10 for natural order on the part kernel). k is the
+    block width the model picks."""
     specs = [("fused64", k, name) for k in range(5, 8) for name in ("greedy", "nat")]
-    specs += [("part", k, "greedy") for k in range(8, 14)]
+    specs += [("part", k, name) for k in range(8, 14) for name in ("greedy", "dfs")]
     specs += [("part", k, "nat") for k in range(5, 11)]
     return specs
 
 
 def build_plan(gates: Sequence[Gate], n: int, spec: Tuple[str, int, str]) -> dict:
     mode, k, name = spec
+    low = range(min(LOW_SLOTS, n))
     if name == "nat":
         parts = nat_parts(gates, k)
+    elif name == "dfs":
+        parts = dfs_parts(gates, max(k, len(low)), always=low) if mode == "part" else dfs_parts(gates, k)
     elif mode == "part":
-        parts = greedy_parts(gates, max(k, min(LOW_SLOTS, n)), always=range(min(LOW_SLOTS, n)))
+        parts = greedy_parts(gates, max(k, len(low)), always=low)
     else:
         parts = greedy_parts(gates, k)
     return {"mode": mode, "k": k, "partitioner": name, "parts": parts}

@@ -67,6 +67,9 @@ struct hsv_state {
   // runs of 2^low_slots amplitudes (512 B for complex128 at 5) sustain
   // 5.0 TB/s on MI355X against 3.9 TB/s for 128 B runs.
   int low_slots = 5;
+  // Low inner bits the register kernel keeps on the lanes in its first and
+  // last layouts (coalesced global loads and stores); 0 turns this off.
+  int io_bits = 6;
   DeviceScratch fused_scratch;       // fused unitary + offset tables
   size_t elem() const { return dtype == HSV_C128 ? 16 : 8; }
 };
@@ -252,28 +255,38 @@ struct RegPlan {
 // Picks the next layout: the targets of the earliest pending non-diagonal
 // gates, topped up with bits of the current layout, then with high bits
 // (low inner bits stay thread bits when possible, for coalesced access).
+// `forbid`: inner bits that must stay thread bits (the first layout keeps
+// the low inner bits on the lanes so the global loads are coalesced); gates
+// needing them are skipped over, not waited for.
 uint32_t choose_layout(const std::vector<int>& pending, const std::vector<uint32_t>& need_bits,
-                       const std::vector<std::array<int, 2>>& need_list, uint32_t cur, int k, int RB) {
+                       const std::vector<std::array<int, 2>>& need_list, uint32_t cur, int k, int RB,
+                       uint32_t forbid = 0) {
   uint32_t want = 0;
   int cnt = 0;
   for (int i : pending) {
     if (!need_bits[i]) continue;
+    if (need_bits[i] & forbid) continue;
     if (cnt + __builtin_popcount(need_bits[i] & ~want) > RB) break;
     for (int t : need_list[i])
       if (t >= 0 && !(want >> t & 1)) { want |= 1u << t; ++cnt; }
     if (cnt == RB) break;
   }
   for (int b = k - 1; b >= 0 && cnt < RB; --b)
-    if ((cur >> b & 1) && !(want >> b & 1)) { want |= 1u << b; ++cnt; }
+    if ((cur >> b & 1) && !(want >> b & 1) && !(forbid >> b & 1)) { want |= 1u << b; ++cnt; }
   for (int b = k - 1; b >= 0 && cnt < RB; --b)
-    if (!(want >> b & 1)) { want |= 1u << b; ++cnt; }
+    if (!(want >> b & 1) && !(forbid >> b & 1)) { want |= 1u << b; ++cnt; }
   return want;
 }
 
 // Splits a part (gates in inner-bit coordinates) into layout segments. Gates
 // may move ahead of later gates only past gates on disjoint qubits, so the
 // result applies the same operator as the part in its given order.
-void plan_register_part(const std::vector<Lowered>& Ls, const int* inner_of, int k, int RB, RegPlan* P) {
+// io_forbid: inner bits the first and the last layout keep as thread bits,
+// so that the global loads and stores of consecutive lanes hit consecutive
+// amplitudes; a layout-only segment is appended when the last layout has
+// them as register bits.
+void plan_register_part(const std::vector<Lowered>& Ls, const int* inner_of, int k, int RB, RegPlan* P,
+                        uint32_t io_forbid) {
   const int ng = (int)Ls.size();
   std::vector<uint32_t> qmask(ng, 0), need_bits(ng, 0);
   std::vector<std::array<int, 2>> need_list(ng, {-1, -1});
@@ -290,7 +303,7 @@ void plan_register_part(const std::vector<Lowered>& Ls, const int* inner_of, int
       }
     pending.push_back(i);
   }
-  uint32_t cur = choose_layout(pending, need_bits, need_list, 0, k, RB);
+  uint32_t cur = choose_layout(pending, need_bits, need_list, 0, k, RB, io_forbid);
   P->segs.clear();
   P->gates.clear();
   P->order.clear();
@@ -331,6 +344,10 @@ void plan_register_part(const std::vector<Lowered>& Ls, const int* inner_of, int
     pending.swap(rest);
     if (!pending.empty()) cur = choose_layout(pending, need_bits, need_list, cur, k, RB);
   } while (!pending.empty());
+  if (cur & io_forbid) {
+    const uint32_t io = choose_layout({}, need_bits, need_list, cur, k, RB, io_forbid);
+    P->segs.push_back(RegSeg{io, (int)P->gates.size(), 0, 0});
+  }
 }
 
 // Register bits and thread bits for a k-qubit part (k >= 8). With complex128,
@@ -403,7 +420,7 @@ int launch_reg_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t par
   int inner_of[64];
   for (int b = 0, r = 0; b < 64; ++b) inner_of[b] = (part_mask >> b & 1) ? r++ : -1;
   RegPlan P;
-  plan_register_part(Ls, inner_of, k, RB, &P);
+  plan_register_part(Ls, inner_of, k, RB, &P, (1u << std::min(s->io_bits, TB)) - 1);
   const uint64_t all = s->n == 64 ? ~0ull : ((1ull << s->n) - 1);
   const uint64_t outer = all & ~part_mask;
   const uint64_t nbases = 1ull << (s->n - k);
@@ -831,6 +848,7 @@ int hsv_tune(hsv_state* s, const char* key, int value) {
   else if (k == "part_kernel" && (value == 0 || value == 1)) s->part_kernel = value;
   else if (k == "reg_bits" && value >= 0 && value <= 5) s->reg_bits = value;
   else if (k == "low_slots" && value >= 0 && value <= 6) s->low_slots = value;
+  else if (k == "io_bits" && value >= 0 && value <= 6) s->io_bits = value;
   else return fail(HSV_ERR_ARG, "unknown tuning key or value out of range: " + k);
   return HSV_OK;
 }

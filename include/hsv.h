@@ -151,7 +151,10 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *                 log2 (2..5; 0 = default for the dtype);
  *   "reg_waves"   persistent workgroups per CU and occupancy slot (1..64);
  *   "low_slots"   lowest qubits added to every LDS-staged part for
- *                 contiguous HBM runs (0..6). */
+ *                 contiguous HBM runs (0..6);
+ *   "io_bits"     low part bits the register kernel keeps on the lanes in
+ *                 its first and last layouts, for coalesced loads and
+ *                 stores (0..6; 0 = off). */
 int hsv_tune(hsv_state* s, const char* key, int value);
 
 /* ---- Batched QAOA MaxCut (not in the reference) ------------------------

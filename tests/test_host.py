@@ -21,7 +21,8 @@ from conftest import ROOT, rand_state
 from gatesets import ALL_ONE, ALL_TWO, params_for, random_circuit
 from hisvsim_amd import capi
 from hisvsim_amd.circuit import Gate, random_layered
-from hisvsim_amd.partition import check_parts, greedy_parts, layered_parts, nat_parts
+from hisvsim_amd.partition import (check_parts, dfs_order, dfs_parts, gate_dag, greedy_parts, layered_parts,
+                                   merge_parts, nat_parts)
 from test_oracle_golden import full_operator, matrix_of
 
 
@@ -145,8 +146,42 @@ def test_partitioners_are_valid(k):
     circuits = [random_layered(14, 6, seed=k), random_circuit(rng, 12, 150, max_controls=1)]
     for gates in circuits:
         kk = max(k, max(len(g.qubits()) for g in gates))
-        for parts in (nat_parts(gates, kk), greedy_parts(gates, kk), layered_parts(gates, 14, max(kk, 3))):
+        for parts in (nat_parts(gates, kk), greedy_parts(gates, kk), layered_parts(gates, 14, max(kk, 3)),
+                      dfs_parts(gates, kk), merge_parts(greedy_parts(gates, kk), kk),
+                      merge_parts(nat_parts(gates, kk), kk)):
             check_parts(gates, parts, max(kk, 3))
+        if kk >= 5:
+            for parts in (nat_parts(gates, kk, always=range(3)), dfs_parts(gates, kk, always=range(3))):
+                check_parts(gates, parts, kk)
+                assert all(len(set().union(*[set(g.qubits()) for g in p]) | {0, 1, 2}) <= kk for p in parts)
+
+
+def test_dfs_order_is_topological_and_depth_first():
+    """Every gate comes after its DAG predecessors; on a circuit of two
+    independent chains, the first chain is followed to its end before the
+    second starts."""
+    rng = np.random.default_rng(5)
+    gates = random_circuit(rng, 10, 200, max_controls=1)
+    order = dfs_order(gates)
+    assert sorted(order) == list(range(len(gates)))
+    pos = {g: i for i, g in enumerate(order)}
+    for i, succ in enumerate(gate_dag(gates)):
+        assert all(pos[i] < pos[j] for j in succ)
+    chains = [Gate("H", (0,)), Gate("H", (2,)), Gate("X", (1,), (0,)), Gate("X", (3,), (2,)), Gate("H", (1,)),
+              Gate("H", (3,))]
+    assert dfs_order(chains) == [0, 2, 4, 1, 3, 5]
+
+
+def test_merge_parts_never_adds_parts():
+    gates = random_layered(20, 10, seed=2)
+    for k in (6, 9, 12):
+        for parts in (nat_parts(gates, k), greedy_parts(gates, k)):
+            merged = merge_parts(parts, k)
+            assert len(merged) <= len(parts)
+            check_parts(gates, merged, k)
+    # neighbours that fit are joined
+    a, b = [Gate("H", (0,))], [Gate("H", (1,))]
+    assert merge_parts([a, b], 2) == [a + b] and merge_parts([a, b], 1) == [a, b]
 
 
 def test_greedy_beats_natural_order_on_layered():
