@@ -4,14 +4,17 @@ on one MI355X, with HBM GB/s of the dominant kernel and the per-gate path
 beside it.
 
     python bench.py [--steps K] [--warmup W] [--n 33] [--depth 40]
-                    [--mode part|fused64|fused32|pergate] [--k K]
+                    [--mode fusedbf16|part|fused64|fused32|pergate|permute] [--k K]
 
 One step = one full simulation of the circuit from |0...0> (state reset
 included) with the chosen path; the state (2^n amplitudes) stays in HBM. The
+default path is the fused-MFMA one: parts of up to 7 qubits fused into FP64
+unitaries and applied with BF16-split products accumulated in FP32. The
 per-gate path is timed on a bounded sample of the same circuit (its first
 layers) and its full time-to-solution is extrapolated per gate. Accuracy of
 each path is checked against the FP64 CPU oracle on the same generator at a
-size the oracle finishes in seconds. Rank 0 prints one JSON line last.
+size the oracle finishes in seconds, and a fused path also against the FP64
+part path at the full size. One JSON line is printed last.
 """
 from __future__ import annotations
 
@@ -74,6 +77,10 @@ def measured_traffic(kernel: str, n: int, mode: str, k: int):
     return None if row is None else row.get("bytes_per_launch")
 
 
+FUSED_MODE = {"fused64": "f64", "fused32": "f32split", "fusedbf16": "bf16split"}
+BF16_MFMA_PEAK = 2.5e15
+
+
 def exchange_permutation(n: int, m: int = 3):
     """The qubit permutation of one exchange step (svsim-mpi.hpp:139-149, a
     stable partition of the slots): qubits 9..9+m-1 move to the top m slots
@@ -100,9 +107,11 @@ def run_path(s: capi.State, mode: str, gates, parts) -> None:
         for p in parts:
             s.apply_part(p, 0)
     else:
-        fm = "f64" if mode == "fused64" else "f32split"
+        fm = FUSED_MODE[mode]
         for p in parts:
             s.apply_fused(p, fm)
+        if fm != "f64":
+            s.renormalize()   # the split-precision products drift the norm by ~1e-5
 
 
 def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0, partitioner: str = "best") -> dict:
@@ -172,19 +181,22 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=0, help="qubits (default 33; 32 for --mode permute, which "
                                                       "needs a second state buffer)")
     ap.add_argument("--depth", type=int, default=40)
-    ap.add_argument("--mode", default="part", choices=["part", "fused64", "fused32", "pergate", "permute"])
+    ap.add_argument("--mode", default="fusedbf16",
+                    choices=["part", "fused64", "fused32", "fusedbf16", "pergate", "permute"])
     ap.add_argument("--k", type=int, default=0, help="part width (default: per mode; permute: qubits exchanged)")
     ap.add_argument("--partitioner", default="best", choices=["best", "greedy", "dfs"],
                     help="acyclic partitioner (best: the one with fewer parts)")
     ap.add_argument("--pergate-layers", type=int, default=1, help="layers of the per-gate timing sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-full-verify", action="store_true",
+                    help="skip the full-size infidelity check of a fused mode against the FP64 part path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU layout is described "
               f"in DESIGN.md and not run here.", file=sys.stderr)
         return 2
-    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "pergate": 1, "permute": 3}[args.mode]
+    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 7, "pergate": 1, "permute": 3}[args.mode]
     n, depth = args.n or (32 if args.mode == "permute" else 33), args.depth
     gates = random_layered(n, depth, seed=0)
     partitioner_used = None
@@ -222,6 +234,22 @@ def main() -> int:
         passes = launches.shape[1]
         log(f"[bench] TTS {tts:.3f} s, {passes} passes/step, mean pass {kern_ms:.3f} ms")
 
+        full_verify = None
+        if args.mode in FUSED_MODE and not args.no_full_verify:
+            # the FP64 register part path on a second full-size state (two
+            # 2^33 complex128 states fit in HBM together): infidelity of the
+            # timed result at the benchmark's own size
+            ref_parts, _ = make_parts(gates, "part", 12, "best")
+            with capi.State(n) as ref:
+                run_path(ref, "part", gates, ref_parts)
+                ref.synchronize()
+                ov = ref.inner(s)
+                full_verify = {"n": n, "reference": "FP64 register part path (k = 12), same circuit",
+                               "infidelity": max(0.0, 1.0 - abs(ov) ** 2 / (ref.norm2() * s.norm2())),
+                               "max_abs_err": ref.max_abs_diff(s), "norm2": s.norm2()}
+            log(f"[bench] full-size check vs FP64 part path: infidelity {full_verify['infidelity']:.3e}, "
+                f"max|d| {full_verify['max_abs_err']:.2e}")
+
         per_gate = None
         if args.mode != "permute":
             # per-gate sample: first layers of the same circuit
@@ -248,14 +276,22 @@ def main() -> int:
                         "tts_extrapolated_s": pergate_tts, "passes": len(gates),
                         "speedup_of_mode": pergate_tts / tts}
 
-    if args.mode in ("part", "pergate", "permute"):
+    if args.mode in ("part", "pergate", "permute", "fusedbf16"):
+        # the BF16-split fused pass executes 3 x 8 x 2^k flops per amplitude
+        # against 32 B of HBM traffic: 96 flop/B at k = 7, under the 312 flop/B
+        # ridge of 2.5 PFLOP/s over 8 TB/s, so it is HBM-bound too
         unit_bytes = 2 * S * 2 ** n
         achieved = unit_bytes / (kern_ms * 1e-3) / 1e9
-        kernel = {"part": "k_part_reg", "pergate": "k_gate", "permute": "k_permute"}[args.mode]
+        kernel = {"part": "k_part_reg", "pergate": "k_gate", "permute": "k_permute",
+                  "fusedbf16": "k_fused_bf"}[args.mode]
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, n, args.mode, k),
                 "kernel": kernel,
                 "per_launch": f"{unit_bytes} B (2 x 16 B x 2^{n}: read + write of the state) / {kern_ms:.3f} ms"}
+        if args.mode == "fusedbf16":
+            kk = max(5, min(7, k))
+            roof["mfma_tflops_algorithmic"] = 8 * 2 ** kk * 2 ** n / (kern_ms * 1e-3) / 1e12
+            roof["mfma_frac_executed"] = 3 * 8 * 2 ** kk * 2 ** n / (kern_ms * 1e-3) / BF16_MFMA_PEAK
     else:
         kk = max(4, min(7, k))
         flops = 8 * 2 ** kk * 2 ** n
@@ -278,7 +314,7 @@ def main() -> int:
     out = {
         "metric": METRIC, "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": tts * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f64" if args.mode != "fused32" else "f32split",
+        "dtype": {"fused32": "f32split", "fusedbf16": "bf16split (fp32 accumulate)"}.get(args.mode, "f64"),
         "data": "synthetic: seeded random layered circuit (U3 angles ~ U[0,2pi), CZ on random matchings), |0..0> input",
         "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, complex128 state "
                                 f"({2 ** n * 16 / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
@@ -292,6 +328,7 @@ def main() -> int:
         "per_gate": per_gate,
         "roofline": roof,
         "verify": verify,
+        "verify_full_size": full_verify,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

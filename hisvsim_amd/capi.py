@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhsv.so")
 
 HSV_C128, HSV_C64 = 0, 1
-HSV_FUSED_F64, HSV_FUSED_F32SPLIT = 0, 1
+HSV_FUSED_F64, HSV_FUSED_F32SPLIT, HSV_FUSED_BF16SPLIT = 0, 1, 2
+FUSED_MODES = {"f64": HSV_FUSED_F64, "f32split": HSV_FUSED_F32SPLIT, "bf16split": HSV_FUSED_BF16SPLIT}
 
 # SvSim::GateType numbering (basic_gates.hpp:12-45), as in hsv.h.
 GATE_TYPES = {
@@ -97,6 +98,7 @@ SIGNATURES = {
     "hsv_probability": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
     "hsv_measure": (ctypes.c_int, [_P, _I, ctypes.c_int, _D, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
     "hsv_project": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
+    "hsv_renormalize": (ctypes.c_int, [_P, _D]),
     "hsv_expect_pauli": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
     "hsv_gate_matrix": (ctypes.c_int, [_G, _D, _I]),
     "hsv_fused_unitary": (ctypes.c_int, [ctypes.c_int, _G, ctypes.c_int, _I, _I, _D, _D]),
@@ -243,7 +245,7 @@ class State:
         _check(lib().hsv_apply_part(self._h, arr, len(gates), int(opt_slots)))
 
     def apply_fused(self, gates: Sequence, mode: str = "f64") -> None:
-        m = {"f64": HSV_FUSED_F64, "f32split": HSV_FUSED_F32SPLIT}[mode]
+        m = FUSED_MODES[mode]
         arr = gate_array(gates)
         _check(lib().hsv_apply_fused(self._h, arr, len(gates), m))
 
@@ -292,6 +294,12 @@ class State:
         _check(lib().hsv_measure(self._h, q, len(qubits), u.ctypes.data_as(_D), len(u),
                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return out
+
+    def renormalize(self) -> float:
+        """Scales the state to unit norm; returns the norm^2 before."""
+        v = ctypes.c_double()
+        _check(lib().hsv_renormalize(self._h, ctypes.byref(v)))
+        return v.value
 
     def project(self, qubits: Sequence[int], values: Sequence[int] = ()) -> float:
         """Collapses the qubits to the values (0 where not given, as the

@@ -25,8 +25,9 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
                 const std::vector<Lowered>& gates, int mode, std::string* err);
 
 // The fused unitary itself (for tests and the cost model): qubits of the
-// part (sorted, padded to at least 4) and the row-major matrix.
+// part (sorted, padded with the lowest free qubits to at least min_qubits)
+// and the row-major matrix.
 int fused_unitary(int n, const std::vector<Lowered>& gates, std::vector<int>* qubits, std::vector<cd>* U,
-                  std::string* err);
+                  std::string* err, int min_qubits = 4);
 
 }  // namespace hsv

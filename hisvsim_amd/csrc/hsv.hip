@@ -699,6 +699,23 @@ int hsv_probability(hsv_state* s, int qubit, int value, double* out) {
   return HSV_OK;
 }
 
+int hsv_renormalize(hsv_state* s, double* norm2_before) {
+  if (!s) return fail(HSV_ERR_ARG, "null handle");
+  double n2 = 0;
+  if (int rc = reduce<RD_NORM2>(s, nullptr, 0, &n2, nullptr)) return rc;
+  if (norm2_before) *norm2_before = n2;
+  if (!(n2 > 0.0)) return fail(HSV_ERR_ARG, "cannot renormalise a zero state");
+  const double scale = 1.0 / std::sqrt(n2);
+  const int grid = grid_for(s->len, 256, 1u << 16);
+  // k_project with an empty mask keeps every amplitude and scales it
+  if (s->dtype == HSV_C128)
+    hipLaunchKernelGGL((k_project<double>), grid, 256, 0, s->stream, amp_as<double>(s), s->len, 0ull, 0ull, scale);
+  else
+    hipLaunchKernelGGL((k_project<float>), grid, 256, 0, s->stream, amp_as<float>(s), s->len, 0ull, 0ull, scale);
+  HIPCHK(hipGetLastError());
+  return HSV_OK;
+}
+
 int hsv_project(hsv_state* s, uint64_t qubit_mask, uint64_t values, double* p_out) {
   if (!s) return fail(HSV_ERR_ARG, "null handle");
   if (qubit_mask >> s->n) return fail(HSV_ERR_ARG, "qubit mask names a qubit beyond the state");

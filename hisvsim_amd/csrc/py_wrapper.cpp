@@ -215,6 +215,13 @@ struct StateVector {
     check(hsv_inner(h, o.h, v), "inner");
     return cd{v[0], v[1]};
   }
+  // Unit norm again (FP64); returns the norm^2 before.
+  double renormalize() {
+    double v = 0;
+    py::gil_scoped_release nogil;
+    check(hsv_renormalize(h, &v), "renormalize");
+    return v;
+  }
   // state_vector.hpp:829-895 (values default to 0), renormalised by
   // 1/sqrt(p); the qubits stay in the state. Returns p.
   double project(const std::vector<int>& qubits, std::vector<int> values) {
@@ -330,6 +337,7 @@ PYBIND11_MODULE(svsimulator_py_wrapper, m) {
       .def("norm2", &StateVector::norm2)
       .def("probability", &StateVector::probability, py::arg("qubit"), py::arg("value") = 0)
       .def("project", &StateVector::project, py::arg("qubits"), py::arg("values") = std::vector<int>{})
+      .def("renormalize", &StateVector::renormalize)
       .def("expectation", &StateVector::expectation, py::arg("x_mask"), py::arg("z_mask"))
       .def("max_abs_diff", &StateVector::max_abs_diff)
       .def("inner", &StateVector::inner)
@@ -368,8 +376,10 @@ PYBIND11_MODULE(svsimulator_py_wrapper, m) {
   m.def(
       "execute_fused",
       [](StateVector& sv, const std::vector<BoundGate*>& gates, const std::string& precision) {
-        int mode = precision == "f64" ? HSV_FUSED_F64 : precision == "f32split" ? HSV_FUSED_F32SPLIT : -1;
-        if (mode < 0) throw py::value_error("precision must be 'f64' or 'f32split'");
+        int mode = precision == "f64" ? HSV_FUSED_F64
+                   : precision == "f32split" ? HSV_FUSED_F32SPLIT
+                   : precision == "bf16split" ? HSV_FUSED_BF16SPLIT : -1;
+        if (mode < 0) throw py::value_error("precision must be 'f64', 'f32split' or 'bf16split'");
         auto v = collect(gates);
         py::gil_scoped_release nogil;
         check(hsv_apply_fused(sv.h, v.data(), (int)v.size(), mode), "execute_fused");

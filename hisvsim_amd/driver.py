@@ -6,7 +6,7 @@ Counterpart of the reference driver's main flow and part executor
 ``execute_qasm`` :1377-1617 for the per-gate path):
 
     python -m hisvsim_amd.driver --app circuit.qasm [--parts --partfile F]
-        [--nat K | --greedy K] [--slots] [--mode part|fused64|fused32|pergate]
+        [--nat K | --greedy K] [--slots] [--mode part|fused64|fused32|fusedbf16|pergate]
         [--lowering reference|textbook] [--dot] [--dump] [--shots N]
         [--n T --alloc A --initn T]   (accepted for compatibility; no effect)
 
@@ -90,8 +90,11 @@ def run(args) -> dict:
             for p in parts:
                 s.apply_part(p, 1 if args.slots else 0)
         else:
+            fm = {"fused64": "f64", "fused32": "f32split", "fusedbf16": "bf16split"}[args.mode]
             for p in parts:
-                s.apply_fused(p, "f64" if args.mode == "fused64" else "f32split")
+                s.apply_fused(p, fm)
+            if fm != "f64":
+                s.renormalize()
         s.synchronize()
         t_sim = time.perf_counter() - t1
         probs = {q: s.probability(q, 1) for q in meas}
@@ -115,7 +118,7 @@ def main(argv=None) -> int:
     ap.add_argument("--nat", type=int, default=0, help="natural-order parts of K qubits")
     ap.add_argument("--greedy", type=int, default=0, help="DAG-aware greedy parts of K qubits")
     ap.add_argument("--slots", action="store_true", help="add the fast low qubits to every part")
-    ap.add_argument("--mode", default="part", choices=["part", "fused64", "fused32", "pergate"])
+    ap.add_argument("--mode", default="part", choices=["part", "fused64", "fused32", "fusedbf16", "pergate"])
     ap.add_argument("--lowering", default="reference", choices=["reference", "textbook"])
     ap.add_argument("--dtype", default="auto", choices=["auto", "complex128", "complex64"],
                     help="state precision (auto: complex128)")

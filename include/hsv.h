@@ -65,7 +65,9 @@ typedef struct hsv_gate {
 /* Fused-part precision modes. */
 enum hsv_fused_mode {
   HSV_FUSED_F64 = 0,        /* FP64 matrix cores (v_mfma_f64)            */
-  HSV_FUSED_F32SPLIT = 1    /* FP32 matrix cores, hi/lo split products   */
+  HSV_FUSED_F32SPLIT = 1,   /* FP32 matrix cores, hi/lo split products   */
+  HSV_FUSED_BF16SPLIT = 2   /* BF16 matrix cores, hi/lo split products
+                               accumulated in FP32; parts of 5..7 qubits */
 };
 
 /* Replaces StateVector(Qubit, int alloc) (state_vector.hpp:60-61, 74-119,
@@ -90,7 +92,8 @@ int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_s
 
 /* Not in the reference: the part's gates fused on the host into one
  * 2^k x 2^k FP64 unitary (checked for unitarity) and applied as a batched
- * complex GEMM on the matrix cores. k <= 7. */
+ * complex GEMM on the matrix cores. k <= 7. HSV_FUSED_BF16SPLIT pads parts
+ * to at least 5 qubits and needs at least 4 state qubits outside the part. */
 int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode);
 
 /* Replaces the qubit exchange of SvSimMPI::gather_qubits / gather_slots
@@ -108,6 +111,12 @@ int hsv_reset(hsv_state* s);
 
 /* FP64 reductions. */
 int hsv_norm2(hsv_state* s, double* out);
+/* Scales the state to unit norm (FP64 norm, one pass); the norm^2 before
+ * scaling goes to *norm2_before (may be NULL). Not in the reference (its
+ * gates preserve the norm in FP64); the split-precision fused paths drift
+ * by ~1e-5 over a circuit and end with this step. A zero state is an error
+ * (HSV_ERR_ARG). */
+int hsv_renormalize(hsv_state* s, double* norm2_before);
 /* <a|b> as (re, im). */
 int hsv_inner(hsv_state* a, hsv_state* b, double* out2);
 /* max |a_i - b_i|; replaces the amplitude loop of operator== and state_equal

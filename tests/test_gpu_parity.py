@@ -128,6 +128,32 @@ def test_fused_f32split_infidelity(orc, k, seed):
     assert np.abs(got - want).max() <= 1e-6
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("k,seed,n", [(3, 4, 12), (5, 5, 14), (6, 6, 15), (7, 7, 16), (7, 8, 20)])
+def test_fused_bf16split_infidelity(orc, k, seed, n, dtype):
+    """BF16-split MFMA path (16 significant bits per operand, three products
+    per real product, FP32 accumulation): infidelity and max error within
+    1e-6 of the FP64 oracle over 6 random parts. Parts under 5 qubits are
+    padded to 5."""
+    rng = np.random.default_rng(700 + seed)
+    parts = random_parts(rng, n, 6, k, 3 * k, names=UNITARY_ONE + UNITARY_TWO, max_controls=1)
+    a = rand_state(n, seed)
+    want = a.copy()
+    for p in parts:
+        orc.apply_gates(want, p)
+    got = gpu_run(n, a, lambda s: [s.apply_fused(p, "bf16split") for p in parts] + [s.renormalize()], dtype)
+    assert abs(np.vdot(got, got).real - 1) <= (1e-12 if dtype == "complex128" else 1e-6)
+    infid = 1 - abs(np.vdot(want, got)) ** 2
+    assert infid <= 1e-6
+    assert np.abs(got - want).max() <= 1e-6
+
+
+def test_fused_bf16split_limits():
+    with capi.State(8) as s:   # 5-qubit part leaves 3 qubits outside: below the 4 the path needs
+        with pytest.raises(capi.HsvError):
+            s.apply_fused([Gate("H", (q,)) for q in range(5)], "bf16split")
+
+
 def test_fused_rejects_non_unitary():
     g = Gate("C2", (0, 1), (), tuple(complex(i + 1, 0) for i in range(16)))
     with capi.State(8) as s:

@@ -113,6 +113,23 @@ def test_gpu_project_matches_reference(i, dtype):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_gpu_renormalize(dtype):
+    from hisvsim_amd import capi
+    n = 10
+    a = rand_state(n, 21)
+    tol = 1e-15 if dtype == "complex128" else 1e-7
+    with capi.State(n, dtype) as s:
+        s.write(0.5 * a)
+        assert abs(s.renormalize() - 0.25) <= 4 * tol
+        assert np.abs(s.read() - a).max() <= 4 * tol
+        assert abs(s.norm2() - 1) <= 10 * tol
+        s.write(np.zeros(2 ** n))
+        with pytest.raises(capi.HsvError):
+            s.renormalize()
+
+
+@pytest.mark.gpu
 def test_gpu_project_zero_probability_is_an_error():
     from hisvsim_amd import capi
     with capi.State(5) as s:          # |00000>: P(qubit 2 = 1) = 0
