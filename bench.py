@@ -32,7 +32,7 @@ sys.path.insert(0, ROOT)
 
 from hisvsim_amd import capi  # noqa: E402
 from hisvsim_amd.circuit import gate_amplitudes, random_layered  # noqa: E402
-from hisvsim_amd.partition import dfs_parts, greedy_parts  # noqa: E402
+from hisvsim_amd.partition import closure_parts, dfs_parts, greedy_parts  # noqa: E402
 
 METRIC = "33q circuit time-to-solution + HBM GB/s; fused-MFMA vs per-gate at ≤1e-6 infid"
 HBM_PEAK_GBS = 8000.0
@@ -48,13 +48,14 @@ LOW_SLOTS = 5  # qubits the device adds to every LDS-staged part (libhsv default
 
 
 def make_parts(gates, mode: str, k: int, partitioner: str = "best"):
-    """Acyclic parts: DAG-aware greedy, depth-first order, or whichever of
-    the two gives fewer parts ("best"). For the register part kernel, k
-    counts the low qubits the device adds to every part anyway, so the
-    partitioner starts each part with them."""
+    """Acyclic parts: DAG-aware greedy, depth-first order, qubit-set
+    closure, or whichever gives the fewest parts ("best"). For the register
+    part kernel, k counts the low qubits the device adds to every part
+    anyway, so the partitioner starts each part with them."""
     always = range(LOW_SLOTS) if mode == "part" else ()
     cands = {"greedy": lambda: greedy_parts(gates, k, always=always),
-             "dfs": lambda: dfs_parts(gates, k, always=always)}
+             "dfs": lambda: dfs_parts(gates, k, always=always),
+             "closure": lambda: closure_parts(gates, k, always=always)}
     if partitioner != "best":
         return cands[partitioner](), partitioner
     return min(((f(), name) for name, f in cands.items()), key=lambda x: len(x[0]))
@@ -184,7 +185,7 @@ def main() -> int:
     ap.add_argument("--mode", default="fusedbf16",
                     choices=["part", "fused64", "fused32", "fusedbf16", "pergate", "permute"])
     ap.add_argument("--k", type=int, default=0, help="part width (default: per mode; permute: qubits exchanged)")
-    ap.add_argument("--partitioner", default="best", choices=["best", "greedy", "dfs"],
+    ap.add_argument("--partitioner", default="best", choices=["best", "greedy", "dfs", "closure"],
                     help="acyclic partitioner (best: the one with fewer parts)")
     ap.add_argument("--pergate-layers", type=int, default=1, help="layers of the per-gate timing sample")
     ap.add_argument("--no-cpu", action="store_true")

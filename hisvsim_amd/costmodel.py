@@ -5,12 +5,13 @@ and the partition; the plain partitioner (natural order, the reference's
 ``*_part_nat`` method) is the fallback, and measured runtime judges the
 model.
 
-Features of a part, for a state of n qubits and a kernel mode (LDS/register
-part kernel or fused FP64 unitary):
+Features of a part, for a state of n qubits and a kernel mode (register
+part kernel, fused FP64 unitary or fused BF16-split unitary):
   part qubits k, the width the device runs it at (k plus the low qubits it
-  adds), gate count, 1-qubit and multi-qubit gate counts, the multi-qubit
-  fraction, min / max / spread of the qubit indices, whether it touches the
-  top 3 (high-order) qubits, the mode and n.
+  adds, or the fused path's padded width), gate count, 1-qubit and
+  multi-qubit gate counts, the multi-qubit fraction, min / max / spread of
+  the qubit indices, whether it touches the top 3 (high-order) qubits, two
+  mode flags and n.
 Target: log(ms per pass scaled to a 2^30-amplitude state), so one model
 covers every n (a pass is memory-bound work proportional to 2^n).
 """
@@ -27,11 +28,12 @@ import numpy as np
 
 from . import capi
 from .circuit import Gate
-from .partition import dfs_parts, greedy_parts, nat_parts
+from .partition import closure_parts, dfs_parts, greedy_parts, nat_parts
 
-MODES = ("part", "fused64")
+MODES = ("part", "fused64", "fusedbf16")
+FUSED_PRECISION = {"fused64": "f64", "fusedbf16": "bf16split"}
 LOW_SLOTS = 5
-N_FEATURES = 12
+N_FEATURES = 13
 PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "costmodel_params.json")
 
 
@@ -41,13 +43,13 @@ def part_features(part: Sequence[Gate], n: int, mode: str) -> np.ndarray:
     if mode == "part":
         width = len(set(qs) | set(range(min(LOW_SLOTS, n))))
     else:
-        width = max(4, k)
+        width = max(4 if mode == "fused64" else 5, k)
     n1 = sum(1 for g in part if len(g.qubits()) == 1)
     n2 = len(part) - n1
     return np.array([
         k / 10.0, width / 10.0, len(part) / 20.0, n1 / 20.0, n2 / 20.0, n2 / max(1, len(part)),
         qs[0] / 30.0, qs[-1] / 30.0, (qs[-1] - qs[0]) / 30.0, float(qs[-1] >= n - 3),
-        float(mode == "fused64"), (n - 25) / 5.0,
+        float(mode == "fused64"), float(mode == "fusedbf16"), (n - 25) / 5.0,
     ], dtype=np.float32)
 
 
@@ -137,13 +139,14 @@ class MLP:
 
 # ---- planning ------------------------------------------------------------------
 def candidate_specs() -> List[Tuple[str, int, str]]:
-    """(mode, k, partitioner) candidates: the fused unitary at k = 5..7 and
-    the part kernel at k = 8..13 (its parts already hold the 5 low qubits),
-    with the DAG-aware greedy, the depth-first or the natural-order
-    partitioner (k = 5..10 for natural order on the part kernel). k is the
-    block width the model picks."""
+    """(mode, k, partitioner) candidates: the fused unitary at k = 5..7 (FP64
+    or BF16-split MFMA) and the part kernel at k = 8..13 (its parts already
+    hold the 5 low qubits), with the DAG-aware greedy, depth-first,
+    qubit-set closure or natural-order partitioner (k = 5..10 for natural
+    order on the part kernel). k is the block width the model picks."""
     specs = [("fused64", k, name) for k in range(5, 8) for name in ("greedy", "nat")]
-    specs += [("part", k, name) for k in range(8, 14) for name in ("greedy", "dfs")]
+    specs += [("fusedbf16", k, name) for k in range(5, 8) for name in ("closure", "dfs")]
+    specs += [("part", k, name) for k in range(8, 14) for name in ("greedy", "dfs", "closure")]
     specs += [("part", k, "nat") for k in range(5, 11)]
     return specs
 
@@ -151,14 +154,13 @@ def candidate_specs() -> List[Tuple[str, int, str]]:
 def build_plan(gates: Sequence[Gate], n: int, spec: Tuple[str, int, str]) -> dict:
     mode, k, name = spec
     low = range(min(LOW_SLOTS, n))
+    fn = {"dfs": dfs_parts, "closure": closure_parts, "greedy": greedy_parts}
     if name == "nat":
         parts = nat_parts(gates, k)
-    elif name == "dfs":
-        parts = dfs_parts(gates, max(k, len(low)), always=low) if mode == "part" else dfs_parts(gates, k)
     elif mode == "part":
-        parts = greedy_parts(gates, max(k, len(low)), always=low)
+        parts = fn[name](gates, max(k, len(low)), always=low)
     else:
-        parts = greedy_parts(gates, k)
+        parts = fn[name](gates, k)
     return {"mode": mode, "k": k, "partitioner": name, "parts": parts}
 
 
@@ -190,10 +192,12 @@ def run_plan(s: capi.State, plan: dict, timed: bool = True) -> np.ndarray:
         s.set_timing(True)
         s.kernel_times()
     for p in plan["parts"]:
-        if plan["mode"] == "fused64":
-            s.apply_fused(p, "f64")
+        if plan["mode"] in FUSED_PRECISION:
+            s.apply_fused(p, FUSED_PRECISION[plan["mode"]])
         else:
             s.apply_part(p, 0)
+    if plan["mode"] == "fusedbf16":
+        s.renormalize()   # not a timed launch: the per-part times stay per part
     s.synchronize()
     if not timed:
         return np.zeros(0)

@@ -68,8 +68,10 @@ struct hsv_state {
   // 5.0 TB/s on MI355X against 3.9 TB/s for 128 B runs.
   int low_slots = 5;
   // Low inner bits the register kernel keeps on the lanes in its first and
-  // last layouts (coalesced global loads and stores); 0 turns this off.
-  int io_bits = 6;
+  // last layouts (coalesced global loads and stores); 0 turns this off. Off
+  // by default: at 31 qubits it measured no gain (the extra layout change
+  // costs what the coalescing saves).
+  int io_bits = 0;
   DeviceScratch fused_scratch;       // fused unitary + offset tables
   size_t elem() const { return dtype == HSV_C128 ? 16 : 8; }
 };

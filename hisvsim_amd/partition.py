@@ -242,6 +242,75 @@ def dfs_parts(gates: Sequence[Gate], k: int, always: Sequence[int] = ()) -> List
     return nat_parts([gates[i] for i in dfs_order(gates)], k, always)
 
 
+def _absorb(qsets: Sequence[frozenset], remaining: Sequence[int], Q: set) -> List[int]:
+    """The blocking scan of ``greedy_parts`` for a fixed qubit set Q: the
+    remaining gates, in order, that act inside Q and follow no skipped gate
+    on a shared qubit."""
+    blocked: set = set()
+    taken: List[int] = []
+    for i in remaining:
+        gq = qsets[i]
+        if gq & blocked or not gq <= Q:
+            blocked |= gq
+            if Q <= blocked:
+                break
+            continue
+        taken.append(i)
+    return taken
+
+
+def closure_parts(gates: Sequence[Gate], k: int, always: Sequence[int] = (),
+                  candidates: int = 24) -> List[List[Gate]]:
+    """Qubit-set-growing clusterer. Each part starts from the ``always``
+    qubits and grows its qubit set Q step by step: a step looks at the
+    qubits that the first ``candidates`` remaining gates not yet inside Q
+    (and not behind a skipped gate) would add, and adds the set with the most
+    absorbed gates per added qubit, a gate being absorbed when the blocking
+    scan of ``greedy_parts`` for Q takes it. The part is that scan's result
+    for the final Q, so the sequence of parts is acyclic for the same reason
+    as ``greedy_parts``. On layered circuits this packs markedly more gates
+    per part than the scan-order methods (an acyclic clusterer in the role
+    of the reference's external dagP partitioner, whose method is not
+    shipped)."""
+    qsets = [frozenset(gate_qubits(g)) for g in gates]
+    base = set(always)
+    if len(base) > k:
+        raise ValueError("more always-included qubits than the part width")
+    remaining = list(range(len(gates)))
+    parts: List[List[Gate]] = []
+    while remaining:
+        Q = set(base)
+        while len(Q) < k:
+            blocked: set = set()
+            cands: List[frozenset] = []
+            for i in remaining:
+                gq = qsets[i]
+                if gq <= Q:
+                    continue
+                add = gq - Q
+                if not (gq & blocked) and len(Q) + len(add) <= k and add not in cands:
+                    cands.append(add)
+                blocked |= gq
+                if len(cands) >= candidates:
+                    break
+            if not cands:
+                break
+            best, best_score = None, -1.0
+            for add in cands:
+                score = len(_absorb(qsets, remaining, Q | add)) / len(add)
+                if score > best_score:
+                    best, best_score = add, score
+            Q |= best
+        taken = _absorb(qsets, remaining, Q)
+        if not taken:
+            raise ValueError(f"gate {gates[remaining[0]]} does not fit a part of {k} qubits "
+                             f"that already holds {sorted(base)}")
+        done = set(taken)
+        parts.append([gates[i] for i in taken])
+        remaining = [i for i in remaining if i not in done]
+    return parts
+
+
 def merge_parts(parts: Sequence[Sequence[Gate]], k: int) -> List[List[Gate]]:
     """Refinement: joins each part with the next while their qubit union
     stays within k. Joining neighbours in the sequence keeps every qubit's

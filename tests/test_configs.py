@@ -1,0 +1,130 @@
+"""The benchmark workloads at their full sizes, checked through properties
+that hold at any size: closed forms where the circuit has one, agreement of
+the FP64 paths to 1e-12, infidelity of the split-precision fused path within
+1e-6, norm preservation. (The CPU oracle covers the same generators at sizes
+it finishes in seconds, in the other test files.)"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import bit_reverse_perm
+from hisvsim_amd import capi
+from hisvsim_amd.circuit import Gate, bernstein_vazirani, gaussian_state, qft, random_layered
+from hisvsim_amd.partition import check_parts, closure_parts
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def fused_infidelity(a: capi.State, b: capi.State) -> float:
+    return 1.0 - abs(a.inner(b)) ** 2 / (a.norm2() * b.norm2())
+
+
+def test_bv24_parts_of_8():
+    """BV, 24 qubits (23 data + ancilla), seed-drawn mask, parts of at most 8
+    qubits: per-gate, register/LDS part and BF16-split fused paths against
+    amp[mask] = 1/sqrt2, amp[mask + 2^23] = -1/sqrt2."""
+    n = 24
+    gates, mask = bernstein_vazirani(n, seed=2024)
+    parts = closure_parts(gates, 8)
+    check_parts(gates, parts, 8)
+    want = np.zeros(2 ** n, complex)
+    want[mask], want[mask + 2 ** (n - 1)] = 1 / math.sqrt(2), -1 / math.sqrt(2)
+    with capi.State(n) as s:
+        s.apply_gates(gates)
+        assert np.abs(s.read() - want).max() <= TOL
+        s.reset()
+        for p in parts:
+            s.apply_part(p, 1)
+        assert np.abs(s.read() - want).max() <= TOL
+        s.reset()
+        for p in closure_parts(gates, 7):
+            s.apply_fused(p, "bf16split")
+        s.renormalize()
+        got = s.read()
+    assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
+
+
+def test_qft24_gaussian_closed_form():
+    """QFT structure on a seeded complex Gaussian input: per-gate FP64 equals
+    the bit-reversal-conjugated inverse DFT (numpy) to 1e-11."""
+    n = 24
+    x = gaussian_state(n, seed=7)
+    rev = bit_reverse_perm(n)
+    want = np.empty_like(x)
+    want[rev] = np.fft.ifft(x[rev]) * math.sqrt(2 ** n)
+    with capi.State(n) as s:
+        s.write(x)
+        s.apply_gates(qft(n))
+        assert np.abs(s.read() - want).max() <= 1e-11
+
+
+def test_qft30_paths_agree():
+    """QFT30 (435 controlled phases, 15 swaps) on a dense input: fused FP64
+    (parts of <= 7 qubits) equals per-gate FP64 to 1e-12, the BF16-split
+    fused path is within 1e-6 infidelity, every path keeps the norm."""
+    n = 30
+    gates = qft(n)
+    prep = [Gate("U3", (q,), (), (0.3 + 0.1 * q, 0.7 * q, 1.1 - 0.05 * q)) for q in range(n)]
+    parts = closure_parts(gates, 7)
+    check_parts(gates, parts, 7)
+    with capi.State(n) as a, capi.State(n) as b:
+        a.apply_gates(prep + gates)
+        b.apply_gates(prep)
+        for p in parts:
+            b.apply_fused(p, "f64")
+        assert a.max_abs_diff(b) <= TOL
+        assert abs(b.norm2() - 1) <= TOL
+        b.reset()
+        b.apply_gates(prep)
+        for p in parts:
+            b.apply_fused(p, "bf16split")
+        b.renormalize()
+        assert fused_infidelity(a, b) <= 1e-6
+
+
+def test_rl33_depth2_part_and_fused_agree():
+    """RL33 (complex128, 137 GB per state) at depth 2: the register part path
+    (parts of 12 including the 5 low qubits) and the fused FP64 path equal
+    the per-gate path to 1e-12 on two full-size states."""
+    n = 33
+    gates = random_layered(n, 2, seed=0)
+    with capi.State(n) as a, capi.State(n) as b:
+        a.apply_gates(gates)
+        for p in closure_parts(gates, 12, always=range(5)):
+            b.apply_part(p, 0)
+        assert a.max_abs_diff(b) <= TOL
+        b.reset()
+        for p in closure_parts(gates, 6):
+            b.apply_fused(p, "f64")
+        assert a.max_abs_diff(b) <= TOL
+
+
+def test_rl34_complex64_depth4():
+    """RL34 in complex64 (137 GB per state): BF16-split fused path against
+    the complex64 per-gate path, infidelity within 1e-6, unit norm."""
+    n = 34
+    gates = random_layered(n, 4, seed=0)
+    with capi.State(n, "complex64") as a, capi.State(n, "complex64") as b:
+        a.apply_gates(gates)
+        for p in closure_parts(gates, 7):
+            b.apply_fused(p, "bf16split")
+        b.renormalize()
+        assert abs(b.norm2() - 1) <= 1e-6
+        assert fused_infidelity(a, b) <= 1e-6
+
+
+def test_qaoa_512_batch_closed_form():
+    """The 512-instance batch (n ~ U{12..22}, 3-regular graphs, p ~ U{1..4})
+    in one hipGraph: every p = 1 instance matches the closed-form <C>."""
+    from hisvsim_amd.qaoa import QAOABatch, expectation_p1, make_batch
+    insts = make_batch(512, seed=0)
+    with QAOABatch(insts) as b:
+        got = b.run()
+    ones = [i for i, x in enumerate(insts) if x.p == 1]
+    assert len(ones) > 50
+    for i in ones:
+        x = insts[i]
+        want = expectation_p1(x.n, x.edges, x.gammas[0], x.betas[0])
+        assert abs(got[i] - want) <= 1e-5 * max(1.0, abs(want))

@@ -38,7 +38,8 @@ def test_features_and_candidates():
     f = part_features(parts[0], n, "part")
     assert f.shape == (N_FEATURES,) and np.all(np.isfinite(f))
     plans = candidate_plans(gates, n)
-    assert {p["mode"] for p in plans} == {"part", "fused64"}
+    assert {p["mode"] for p in plans} == {"part", "fused64", "fusedbf16"}
+    assert {p["partitioner"] for p in plans} == {"greedy", "nat", "dfs", "closure"}
     for p in plans:
         check_parts(gates, p["parts"])
     check_parts(gates, fallback_plan(gates, n)["parts"], 10)
@@ -75,3 +76,37 @@ def test_gpu_training_fits_and_is_deterministic():
     assert la == lb and np.array_equal(a.params(), b.params())
     a.close()
     b.close()
+
+
+def test_committed_model_matches_features():
+    """The shipped parameters are for this feature vector and network."""
+    import json
+    from hisvsim_amd.costmodel import PARAMS_FILE
+    d = json.load(open(PARAMS_FILE))
+    assert d["in"] == N_FEATURES
+    h = d["hidden"]
+    assert len(d["params"]) == N_FEATURES * h + h + h * h + h + h + 1
+
+
+@pytest.mark.gpu
+def test_trained_model_beats_plain_partitioner():
+    """What the learned part buys: on a 28-qubit random layered circuit the
+    plan the committed model picks runs faster than the plain partitioner
+    (natural order, k = 10), and the two plans give the same state to the
+    picked plan's precision."""
+    from hisvsim_amd import capi
+    from hisvsim_amd.costmodel import MLP, choose_plan, fallback_plan, run_plan
+    n = 28
+    gates = random_layered(n, 30, seed=11)
+    model = MLP.load()
+    best, plans = choose_plan(model, gates, n)
+    fb = fallback_plan(gates, n)
+    with capi.State(n) as a, capi.State(n) as b:
+        run_plan(a, best)      # warm-up (code objects, scratch)
+        a.reset()
+        t_model = float(run_plan(a, best).sum())
+        t_fb = float(run_plan(b, fb).sum())
+        infid = 1 - abs(a.inner(b)) ** 2
+    model.close()
+    assert t_model < 0.8 * t_fb, (best["mode"], best["k"], best["partitioner"], t_model, t_fb)
+    assert infid <= (1e-12 if best["mode"] != "fusedbf16" else 1e-6)
