@@ -166,7 +166,13 @@ struct BfArgs {
 
 constexpr int kBfThreads = 512;
 
-template <typename R, int K>
+// SWZ: row order of the B operands and of the results. false: lane l, element
+// e reads row 8*(l/16) + e of its k-step and stores result item i to row
+// 4*(l/16) + i. true: rows (l/16) + 4*e and 4*i + (l/16), so the 4 lanes of
+// a column cover 4 consecutive rows in each load and store instruction; the
+// host permutes U's columns and rows to match (any k order shared by A and
+// B, and any row order of A, leave the product the same up to that order).
+template <typename R, int K, bool SWZ>
 __global__ __launch_bounds__(kBfThreads) void k_fused_bf(typename VecT<R>::type* __restrict__ a,
                                                        const bf8* __restrict__ ufrag,
                                                        const uint64_t* __restrict__ qoff,
@@ -196,7 +202,7 @@ __global__ __launch_bounds__(kBfThreads) void k_fused_bf(typename VecT<R>::type*
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-      for (int e = 0; e < 8; ++e) raw[ks][e] = a[base + sq[32 * ks + 8 * kgrp + e]];
+      for (int e = 0; e < 8; ++e) raw[ks][e] = a[base + sq[32 * ks + (SWZ ? kgrp + 4 * e : 8 * kgrp + e)]];
   }
   for (; cb < f.nblocks; cb += nw) {
     const uint64_t base = pdep64(cb, f.outer_mask) + coff;
@@ -218,7 +224,7 @@ __global__ __launch_bounds__(kBfThreads) void k_fused_bf(typename VecT<R>::type*
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) raw[ks][e] = a[nbase + sq[32 * ks + 8 * kgrp + e]];
+        for (int e = 0; e < 8; ++e) raw[ks][e] = a[nbase + sq[32 * ks + (SWZ ? kgrp + 4 * e : 8 * kgrp + e)]];
     }
     // every row of this column block is in registers now: results can be
     // written as each row block finishes
@@ -248,7 +254,7 @@ __global__ __launch_bounds__(kBfThreads) void k_fused_bf(typename VecT<R>::type*
         V o;
         o.x = (R)(accR[i] - accT[i]);
         o.y = (R)accI[i];
-        a[base + sq[16 * rb + 4 * kgrp + i]] = o;
+        a[base + sq[16 * rb + (SWZ ? 4 * i + kgrp : 4 * kgrp + i)]] = o;
       }
     }
   }
@@ -340,14 +346,14 @@ int ensure_scratch(DeviceScratch* scratch, size_t bytes, hipStream_t stream, std
   return HSV_OK;
 }
 
-template <typename R, int K>
+template <typename R, int K, bool SWZ>
 void launch_bf(void* amp, const void* scratch, int grid, hipStream_t st, const BfArgs& f, int* rc) {
   constexpr int D = 1 << K;
   const size_t frag_bytes = size_t(8) * D * D;
   const auto* U = static_cast<const bf8*>(scratch);
   const uint64_t* qoff = reinterpret_cast<const uint64_t*>(static_cast<const unsigned char*>(scratch) + frag_bytes);
   const uint64_t* toff = qoff + D;
-  auto kern = &k_fused_bf<R, K>;
+  auto kern = &k_fused_bf<R, K, SWZ>;
   if (frag_bytes > 65536 &&
       hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)frag_bytes) != hipSuccess) {
@@ -361,7 +367,7 @@ void launch_bf(void* amp, const void* scratch, int grid, hipStream_t st, const B
 // The BF16-split path: U as BF16 hi/lo fragments in the MFMA A-operand order,
 // columns in blocks of 16 along the lowest 4 slots outside the part.
 int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
-                   const std::vector<int>& Q, const std::vector<cd>& Um, std::string* err) {
+                   const std::vector<int>& Q, const std::vector<cd>& Um, int swizzle, std::string* err) {
   const int K = (int)Q.size();
   const int D = 1 << K;
   if (K < 5 || K > 7) {
@@ -376,7 +382,10 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
     for (int ks = 0; ks < KS; ++ks)
       for (int lane = 0; lane < 64; ++lane)
         for (int e = 0; e < 8; ++e) {
-          const cd u = Um[size_t(16 * rb + lane % 16) * D + 32 * ks + 8 * (lane / 16) + e];
+          const int mm = lane % 16, g = lane / 16;
+          const int row = 16 * rb + (swizzle ? 4 * (mm % 4) + mm / 4 : mm);
+          const int col = 32 * ks + (swizzle ? g + 4 * e : 8 * g + e);
+          const cd u = Um[size_t(row) * D + col];
           const uint16_t rh = bf16_rne((float)u.real()), ih = bf16_rne((float)u.imag());
           const uint16_t rl = bf16_rne((float)(u.real() - bf16_value(rh)));
           const uint16_t il = bf16_rne((float)(u.imag() - bf16_value(ih)));
@@ -407,14 +416,15 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
   const int grid = (int)std::min<uint64_t>((f.nblocks + waves - 1) / waves, uint64_t(num_cu) * 2);
   int rc = HSV_OK;
   const bool c128 = dtype == HSV_C128;
-  switch (K) {
-    case 5: c128 ? launch_bf<double, 5>(amp, scratch->ptr, grid, stream, f, &rc)
-                 : launch_bf<float, 5>(amp, scratch->ptr, grid, stream, f, &rc); break;
-    case 6: c128 ? launch_bf<double, 6>(amp, scratch->ptr, grid, stream, f, &rc)
-                 : launch_bf<float, 6>(amp, scratch->ptr, grid, stream, f, &rc); break;
-    default: c128 ? launch_bf<double, 7>(amp, scratch->ptr, grid, stream, f, &rc)
-                  : launch_bf<float, 7>(amp, scratch->ptr, grid, stream, f, &rc); break;
+#define HSV_BF_LAUNCH(K_, S_)                                                                  \
+  (c128 ? launch_bf<double, K_, S_>(amp, scratch->ptr, grid, stream, f, &rc)                  \
+        : launch_bf<float, K_, S_>(amp, scratch->ptr, grid, stream, f, &rc))
+  if (swizzle) {
+    if (K == 5) HSV_BF_LAUNCH(5, true); else if (K == 6) HSV_BF_LAUNCH(6, true); else HSV_BF_LAUNCH(7, true);
+  } else {
+    if (K == 5) HSV_BF_LAUNCH(5, false); else if (K == 6) HSV_BF_LAUNCH(6, false); else HSV_BF_LAUNCH(7, false);
   }
+#undef HSV_BF_LAUNCH
   if (rc) { *err = "bf16 fused kernel attribute setup failed"; return rc; }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -427,7 +437,7 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
 }  // namespace
 
 int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
-                const std::vector<Lowered>& gates, int mode, std::string* err) {
+                const std::vector<Lowered>& gates, int mode, std::string* err, int bf_swizzle) {
   if (mode != HSV_FUSED_F64 && mode != HSV_FUSED_F32SPLIT && mode != HSV_FUSED_BF16SPLIT) {
     *err = "unknown fused mode";
     return HSV_ERR_ARG;
@@ -447,7 +457,8 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
     *err = "fused parts need at least 4 state qubits outside the part";
     return HSV_ERR_LIMIT;
   }
-  if (mode == HSV_FUSED_BF16SPLIT) return fused_apply_bf(amp, n, dtype, stream, num_cu, scratch, Q, Um, err);
+  if (mode == HSV_FUSED_BF16SPLIT)
+    return fused_apply_bf(amp, n, dtype, stream, num_cu, scratch, Q, Um, bf_swizzle, err);
   int NC = 2048 / (16 * D);
   while (NC > 1 && 4 + __builtin_ctz(NC) > free_bits) NC >>= 1;
   const int C = 16 * NC;

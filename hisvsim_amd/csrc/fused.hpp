@@ -20,9 +20,11 @@ struct DeviceScratch {
 
 // Applies the fused unitary of `gates` to the n-qubit state at `amp`
 // (dtype HSV_C128 or HSV_C64) on `stream`. mode: hsv_fused_mode.
-// `num_cu` sizes the persistent grid. Returns 0 or a negative hsv_status.
+// `num_cu` sizes the persistent grid. bf_swizzle selects the row order of
+// the BF16-split kernel's loads and stores (1: four consecutive rows per
+// column in each instruction). Returns 0 or a negative hsv_status.
 int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
-                const std::vector<Lowered>& gates, int mode, std::string* err);
+                const std::vector<Lowered>& gates, int mode, std::string* err, int bf_swizzle = 1);
 
 // The fused unitary itself (for tests and the cost model): qubits of the
 // part (sorted, padded with the lowest free qubits to at least min_qubits)

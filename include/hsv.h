@@ -163,7 +163,10 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *                 contiguous HBM runs (0..6);
  *   "io_bits"     low part bits the register kernel keeps on the lanes in
  *                 its first and last layouts, for coalesced loads and
- *                 stores (0..6; 0 = off, the default). */
+ *                 stores (0..6; 0 = off, the default);
+ *   "fused_swizzle" row order of the BF16-split fused kernel's loads and
+ *                 stores (1, the default: the 4 lanes of a column cover 4
+ *                 consecutive rows per instruction; 0: 8 rows per lane). */
 int hsv_tune(hsv_state* s, const char* key, int value);
 
 /* ---- Batched QAOA MaxCut (not in the reference) ------------------------
