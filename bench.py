@@ -52,7 +52,9 @@ def make_parts(gates, mode: str, k: int, partitioner: str = "best"):
     closure, or whichever gives the fewest parts ("best"). For the register
     part kernel, k counts the low qubits the device adds to every part
     anyway, so the partitioner starts each part with them."""
-    always = range(LOW_SLOTS) if mode == "part" else ()
+    # the register kernel and the wide (k >= 8) BF16-split kernel load whole
+    # parts as tiles, so their parts start with the low qubits for long runs
+    always = range(LOW_SLOTS) if mode == "part" or (mode == "fusedbf16" and k >= 8) else ()
     cands = {"greedy": lambda: greedy_parts(gates, k, always=always),
              "dfs": lambda: dfs_parts(gates, k, always=always),
              "closure": lambda: closure_parts(gates, k, always=always)}
@@ -197,7 +199,7 @@ def main() -> int:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU layout is described "
               f"in DESIGN.md and not run here.", file=sys.stderr)
         return 2
-    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 7, "pergate": 1, "permute": 3}[args.mode]
+    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 12, "pergate": 1, "permute": 3}[args.mode]
     n, depth = args.n or (32 if args.mode == "permute" else 33), args.depth
     gates = random_layered(n, depth, seed=0)
     partitioner_used = None
@@ -278,21 +280,28 @@ def main() -> int:
                         "speedup_of_mode": pergate_tts / tts}
 
     if args.mode in ("part", "pergate", "permute", "fusedbf16"):
-        # the BF16-split fused pass executes 3 x 8 x 2^k flops per amplitude
-        # against 32 B of HBM traffic: 96 flop/B at k = 7, under the 312 flop/B
-        # ridge of 2.5 PFLOP/s over 8 TB/s, so it is HBM-bound too
+        # a BF16-split fused pass executes 3 x 8 x 2^w flops per amplitude for
+        # each fused block of width w (one block of k <= 7 qubits, or steps of
+        # 5 qubits inside a wide part) against 32 B of HBM traffic: at most
+        # ~100 flop/B per block, under the 312 flop/B ridge of 2.5 PFLOP/s
+        # over 8 TB/s, so the pass is HBM-bound too
         unit_bytes = 2 * S * 2 ** n
         achieved = unit_bytes / (kern_ms * 1e-3) / 1e9
         kernel = {"part": "k_part_reg", "pergate": "k_gate", "permute": "k_permute",
-                  "fusedbf16": "k_fused_bf"}[args.mode]
+                  "fusedbf16": "k_part_bf" if k >= 8 else "k_fused_bf"}[args.mode]
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, n, args.mode, k),
                 "kernel": kernel,
                 "per_launch": f"{unit_bytes} B (2 x 16 B x 2^{n}: read + write of the state) / {kern_ms:.3f} ms"}
         if args.mode == "fusedbf16":
-            kk = max(5, min(7, k))
-            roof["mfma_tflops_algorithmic"] = 8 * 2 ** kk * 2 ** n / (kern_ms * 1e-3) / 1e12
-            roof["mfma_frac_executed"] = 3 * 8 * 2 ** kk * 2 ** n / (kern_ms * 1e-3) / BF16_MFMA_PEAK
+            # blocks per pass: 1 for k <= 7; for wide parts the 5-qubit steps the
+            # host cuts (the closure method of partition.closure_parts, k = 5)
+            blocks = ([len(closure_parts(p, 5)) for p in parts] if k >= 8 else [1] * len(parts))
+            width = 5 if k >= 8 else max(5, min(7, k))
+            per_pass = float(np.mean(blocks)) * 8 * 2 ** width * 2 ** n
+            roof["fused_blocks_per_pass"] = float(np.mean(blocks))
+            roof["mfma_tflops_algorithmic"] = per_pass / (kern_ms * 1e-3) / 1e12
+            roof["mfma_frac_executed"] = 3 * per_pass / (kern_ms * 1e-3) / BF16_MFMA_PEAK
     else:
         kk = max(4, min(7, k))
         flops = 8 * 2 ** kk * 2 ** n

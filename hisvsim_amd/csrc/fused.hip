@@ -260,6 +260,265 @@ __global__ __launch_bounds__(kBfThreads) void k_fused_bf(typename VecT<R>::type*
   }
 }
 
+// ---- BF16-split fused path, U-stationary tile form ---------------------------------
+// A workgroup (8 waves) owns a tile of 4096 amplitudes: the part's D rows by
+// C = 4096 / D columns along the lowest slots outside the part. Tile element
+// t (12 bits) maps to the tile's 12 slots in ascending order, so consecutive
+// threads load and store consecutive amplitudes. Loaded values are split to
+// BF16 and scattered into B-operand fragments in LDS; wave w keeps row block
+// w % RB of U in registers (its A fragments) and computes that row block for
+// its share of the columns; results go through an FP32 staging tile in LDS
+// and are stored back coalesced. LDS: 32 KB of fragments + 32 KB of staging.
+struct BfTileTables {
+  uint64_t alo[64], ahi[64];    // address offset of tile index bits 0-5 / 6-11
+  uint16_t rlo[64], rhi[64];    // row index contribution of those bits
+  uint16_t clo[64], chi[64];    // column index contribution
+  uint16_t tr[128], tc[128];    // tile index of row r / column c (OR-ed)
+};
+
+constexpr int kBfTileThreads = 512;
+
+template <typename R, int K>
+__global__ __launch_bounds__(kBfTileThreads) void k_fused_bft(typename VecT<R>::type* __restrict__ a,
+                                                            const bf8* __restrict__ ufrag,
+                                                            const BfTileTables* __restrict__ tabs,
+                                                            uint64_t ntiles, uint64_t outer_mask) {
+  constexpr int D = 1 << K;
+  constexpr int RB = D / 16, KS = D / 32;
+  constexpr int C = 4096 / D, NCB = C / 16;
+  constexpr int W = kBfTileThreads / 64;
+  constexpr int CG = W / RB;        // waves sharing one row block
+  constexpr int CBW = NCB / CG;     // column blocks per wave
+  constexpr int PER = 4096 / kBfTileThreads;
+  using V = typename VecT<R>::type;
+  __shared__ bf8 xs[KS * NCB * 4 * 64];   // [ks][cb][set][lane], set = Xr_h, Xr_l, Xi_h, Xi_l
+  __shared__ float2 ys[4096];             // results in tile order
+  __shared__ BfTileTables tb;
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(tabs);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&tb);
+    for (int i = threadIdx.x; i < (int)(sizeof(BfTileTables) / 4); i += kBfTileThreads) dst[i] = src[i];
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int rb = wave % RB, cg = wave / RB;
+  bf8 A[KS][4];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) A[ks][s] = ufrag[((rb * KS + ks) * 4 + s) * 64 + lane];
+  __syncthreads();
+  __bf16* xb = reinterpret_cast<__bf16*>(xs);
+  typedef float v4f_t __attribute__((ext_vector_type(4)));
+  V raw[PER];
+  uint64_t tile = blockIdx.x;
+  if (tile < ntiles) {
+    const uint64_t base = pdep64(tile, outer_mask);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int t = threadIdx.x + kBfTileThreads * j;
+      raw[j] = a[base + tb.alo[t & 63] + tb.ahi[t >> 6]];
+    }
+  }
+  for (; tile < ntiles; tile += gridDim.x) {
+    const uint64_t base = pdep64(tile, outer_mask);
+    // split and scatter this thread's elements into the B fragments:
+    // row r = 32 ks + g + 4 e goes to lane 16 g + (c % 16), element e
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int t = threadIdx.x + kBfTileThreads * j;
+      const int r = tb.rlo[t & 63] + tb.rhi[t >> 6];
+      const int c = tb.clo[t & 63] + tb.chi[t >> 6];
+      const int ks = r >> 5, g = r & 3, e = (r >> 2) & 7;
+      const int frag = (ks * NCB + (c >> 4)) * 4 * 64 + 16 * g + (c & 15);
+      const float xr = (float)raw[j].x, xi = (float)raw[j].y;
+      const __bf16 hr = (__bf16)xr, hi = (__bf16)xi;
+      xb[(frag + 0 * 64) * 8 + e] = hr;
+      xb[(frag + 1 * 64) * 8 + e] = (__bf16)(xr - (float)hr);
+      xb[(frag + 2 * 64) * 8 + e] = hi;
+      xb[(frag + 3 * 64) * 8 + e] = (__bf16)(xi - (float)hi);
+    }
+    const uint64_t tn = tile + gridDim.x;
+    if (tn < ntiles) {
+      const uint64_t nbase = pdep64(tn, outer_mask);
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        const int t = threadIdx.x + kBfTileThreads * j;
+        raw[j] = a[nbase + tb.alo[t & 63] + tb.ahi[t >> 6]];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < CBW; ++w) {
+      const int cb = cg * CBW + w;
+      v4f_t accR = {0, 0, 0, 0}, accT = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const bf8* B = xs + (ks * NCB + cb) * 4 * 64 + lane;
+        const bf8 xrh = B[0], xrl = B[64], xih = B[128], xil = B[192];
+        accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][0], xrh, accR, 0, 0, 0);
+        accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][0], xrl, accR, 0, 0, 0);
+        accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][1], xrh, accR, 0, 0, 0);
+        accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][2], xih, accT, 0, 0, 0);
+        accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][2], xil, accT, 0, 0, 0);
+        accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][3], xih, accT, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][0], xih, accI, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][0], xil, accI, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][1], xih, accI, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][2], xrh, accI, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][2], xrl, accI, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[ks][3], xrh, accI, 0, 0, 0);
+      }
+      // item i of lane l: row 16 rb + 4 i + l/16 (the swizzled A rows), column 16 cb + l % 16
+      const int col = 16 * cb + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = 16 * rb + 4 * i + (lane >> 4);
+        ys[tb.tr[row] | tb.tc[col]] = make_float2(accR[i] - accT[i], accI[i]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int t = threadIdx.x + kBfTileThreads * j;
+      const float2 y = ys[t];
+      V o;
+      o.x = (R)y.x;
+      o.y = (R)y.y;
+      a[base + tb.alo[t & 63] + tb.ahi[t >> 6]] = o;
+    }
+  }
+}
+
+// ---- BF16-split wide parts: 12- or 13-qubit tiles, 5-qubit fused steps ------------
+// For a part of 8..13 qubits the host splits the gates into steps of at most
+// 5 qubits (the qubit-set closure method), fuses each step into a 32 x 32
+// FP64 unitary and sends its BF16 A fragments. A workgroup owns one tile of
+// 2^KT amplitudes (the part's qubits, padded with the lowest free ones, in
+// ascending order: tile index bit j <-> the j-th of them): it loads the tile
+// coalesced into LDS as FP32, applies the steps there one after another
+// (rows = the step's 5 tile bits, columns = the other KT - 5; each wave owns
+// column blocks of 16 and updates them in place; a barrier between steps),
+// and stores the tile back coalesced. One HBM pass per part, the gate work
+// on the matrix cores.
+struct PartBfStep {
+  uint16_t rt[32];              // tile index bits of step row r
+  uint16_t ctl[16], cth[16];    // of step column c: ctl[c & 15] | cth[c >> 4]
+};
+struct PartBfTables {
+  uint64_t alo[64], ahi[128];   // address offset of tile index bits 0-5 / 6-12
+};
+constexpr int kPartBfThreads = 512;
+constexpr int kPartBfMaxSteps = 12;
+
+template <typename R, int KT>
+__global__ __launch_bounds__(kPartBfThreads) void k_part_bf(typename VecT<R>::type* __restrict__ a,
+                                                          const PartBfTables* __restrict__ tabs,
+                                                          const PartBfStep* __restrict__ steps_g,
+                                                          const bf8* __restrict__ ufrag, int nsteps,
+                                                          uint64_t ntiles, uint64_t outer_mask) {
+  constexpr int T = 1 << KT;
+  constexpr int NCB = (1 << (KT - 5)) / 16;   // column blocks of a step
+  constexpr int W = kPartBfThreads / 64;
+  constexpr int CBW = NCB / W;                // column blocks per wave
+  constexpr int PER = T / kPartBfThreads;
+  using V = typename VecT<R>::type;
+  typedef float v4f_t __attribute__((ext_vector_type(4)));
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  bf8* su = reinterpret_cast<bf8*>(smem_raw);                           // [step][rb][set][lane]
+  __shared__ float2 ys[T];
+  __shared__ PartBfStep st[kPartBfMaxSteps];
+  __shared__ uint64_t alo[64], ahi[128];
+  for (int i = threadIdx.x; i < nsteps * 2 * 4 * 64; i += kPartBfThreads) su[i] = ufrag[i];
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(steps_g);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(st);
+    for (int i = threadIdx.x; i < nsteps * (int)(sizeof(PartBfStep) / 4); i += kPartBfThreads) dst[i] = src[i];
+  }
+  for (int i = threadIdx.x; i < 64; i += kPartBfThreads) alo[i] = tabs->alo[i];
+  for (int i = threadIdx.x; i < 128; i += kPartBfThreads) ahi[i] = tabs->ahi[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane >> 4, cc = lane & 15;
+  V raw[PER];
+  uint64_t tile = blockIdx.x;
+  if (tile < ntiles) {
+    const uint64_t base = pdep64(tile, outer_mask);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int t = threadIdx.x + kPartBfThreads * j;
+      raw[j] = a[base + alo[t & 63] + ahi[t >> 6]];
+    }
+  }
+  for (; tile < ntiles; tile += gridDim.x) {
+    const uint64_t base = pdep64(tile, outer_mask);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) ys[threadIdx.x + kPartBfThreads * j] = make_float2((float)raw[j].x, (float)raw[j].y);
+    const uint64_t tn = tile + gridDim.x;
+    if (tn < ntiles) {
+      const uint64_t nbase = pdep64(tn, outer_mask);
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        const int t = threadIdx.x + kPartBfThreads * j;
+        raw[j] = a[nbase + alo[t & 63] + ahi[t >> 6]];
+      }
+    }
+    for (int s = 0; s < nsteps; ++s) {
+      __syncthreads();  // the tile (or the previous step's results) is in LDS
+      const PartBfStep& S = st[s];
+      const bf8* As = su + s * 2 * 4 * 64 + lane;
+#pragma unroll
+      for (int w = 0; w < CBW; ++w) {
+        const int col = 16 * (wave * CBW + w) + cc;
+        const int tc = S.ctl[col & 15] | S.cth[col >> 4];
+        bf8 xrh, xrl, xih, xil;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float2 v = ys[S.rt[g + 4 * e] | tc];
+          const __bf16 hr = (__bf16)v.x, hi = (__bf16)v.y;
+          xrh[e] = hr;
+          xrl[e] = (__bf16)(v.x - (float)hr);
+          xih[e] = hi;
+          xil[e] = (__bf16)(v.y - (float)hi);
+        }
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          const bf8* A = As + rb * 4 * 64;
+          const bf8 urh = A[0], url = A[64], uih = A[128], uil = A[192];
+          v4f_t accR = {0, 0, 0, 0}, accT = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
+          accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrh, accR, 0, 0, 0);
+          accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrl, accR, 0, 0, 0);
+          accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xrh, accR, 0, 0, 0);
+          accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xih, accT, 0, 0, 0);
+          accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xil, accT, 0, 0, 0);
+          accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xih, accT, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xih, accI, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xil, accI, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xih, accI, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrh, accI, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrl, accI, 0, 0, 0);
+          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xrh, accI, 0, 0, 0);
+          // this wave read every row of its column block above: write in place
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            ys[S.rt[16 * rb + 4 * i + g] | tc] = make_float2(accR[i] - accT[i], accI[i]);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int t = threadIdx.x + kPartBfThreads * j;
+      const float2 y = ys[t];
+      V o;
+      o.x = (R)y.x;
+      o.y = (R)y.y;
+      a[base + alo[t & 63] + ahi[t >> 6]] = o;
+    }
+  }
+}
+
 namespace {
 
 uint64_t pdep_host(uint64_t x, uint64_t mask) {
@@ -366,17 +625,269 @@ void launch_bf(void* amp, const void* scratch, int grid, hipStream_t st, const B
 
 // The BF16-split path: U as BF16 hi/lo fragments in the MFMA A-operand order,
 // columns in blocks of 16 along the lowest 4 slots outside the part.
+template <typename R, int K>
+void launch_bft(void* amp, const void* scratch, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer) {
+  const size_t frag_bytes = size_t(8) << (2 * K);
+  const auto* U = static_cast<const bf8*>(scratch);
+  const auto* T = reinterpret_cast<const BfTileTables*>(static_cast<const unsigned char*>(scratch) + frag_bytes);
+  hipLaunchKernelGGL((k_fused_bft<R, K>), grid, kBfTileThreads, 0, st,
+                     static_cast<typename VecT<R>::type*>(amp), U, T, ntiles, outer);
+}
+
+// Tables of the tile form: tile index bit j <-> the j-th lowest of the tile's
+// 12 slots (the part's qubits plus the 12 - K lowest slots outside it).
+void build_tile_tables(uint64_t qmask, uint64_t cmask, BfTileTables* T) {
+  int slot[12], rowpos[12], colpos[12];
+  int j = 0, nr = 0, nc = 0;
+  for (int b = 0; b < 64 && j < 12; ++b) {
+    if (qmask >> b & 1) { slot[j] = b; rowpos[j] = nr++; colpos[j] = -1; ++j; }
+    else if (cmask >> b & 1) { slot[j] = b; rowpos[j] = -1; colpos[j] = nc++; ++j; }
+  }
+  std::memset(T, 0, sizeof(*T));
+  for (int v = 0; v < 64; ++v)
+    for (int bit = 0; bit < 6; ++bit) {
+      if (!(v >> bit & 1)) continue;
+      T->alo[v] += 1ull << slot[bit];
+      T->ahi[v] += 1ull << slot[6 + bit];
+      if (rowpos[bit] >= 0) T->rlo[v] += (uint16_t)(1u << rowpos[bit]);
+      if (rowpos[6 + bit] >= 0) T->rhi[v] += (uint16_t)(1u << rowpos[6 + bit]);
+      if (colpos[bit] >= 0) T->clo[v] += (uint16_t)(1u << colpos[bit]);
+      if (colpos[6 + bit] >= 0) T->chi[v] += (uint16_t)(1u << colpos[6 + bit]);
+    }
+  for (int q = 0; q < 12; ++q) {
+    for (int r = 0; r < (1 << nr); ++r)
+      if (rowpos[q] >= 0 && (r >> rowpos[q] & 1)) T->tr[r] |= (uint16_t)(1u << q);
+    for (int c = 0; c < (1 << nc); ++c)
+      if (colpos[q] >= 0 && (c >> colpos[q] & 1)) T->tc[c] |= (uint16_t)(1u << q);
+  }
+}
+
+// Qubit-set closure over gate qubit masks (the method of
+// hisvsim_amd/partition.py closure_parts): each group grows its qubit set by
+// the candidate that absorbs the most gates per added qubit, a gate being
+// absorbed when no earlier skipped gate shares a qubit with it. Groups run
+// in order; every qubit keeps its gate order.
+std::vector<std::vector<int>> closure_groups(const std::vector<uint64_t>& qm, int k) {
+  auto absorb = [&](const std::vector<int>& rem, uint64_t Q) {
+    std::vector<int> taken;
+    uint64_t blocked = 0;
+    for (int i : rem) {
+      const uint64_t g = qm[i];
+      if ((g & blocked) || (g & ~Q)) {
+        blocked |= g;
+        if ((Q & ~blocked) == 0) break;
+        continue;
+      }
+      taken.push_back(i);
+    }
+    return taken;
+  };
+  std::vector<int> rem(qm.size());
+  for (size_t i = 0; i < qm.size(); ++i) rem[i] = (int)i;
+  std::vector<std::vector<int>> out;
+  while (!rem.empty()) {
+    uint64_t Q = 0;
+    while (__builtin_popcountll(Q) < k) {
+      uint64_t blocked = 0;
+      std::vector<uint64_t> cands;
+      for (int i : rem) {
+        const uint64_t g = qm[i];
+        if ((g & ~Q) == 0) continue;
+        const uint64_t add = g & ~Q;
+        if (!(g & blocked) && __builtin_popcountll(Q | g) <= k &&
+            std::find(cands.begin(), cands.end(), add) == cands.end())
+          cands.push_back(add);
+        blocked |= g;
+        if (cands.size() >= 24) break;
+      }
+      if (cands.empty()) break;
+      uint64_t best = 0;
+      double best_score = -1;
+      for (uint64_t add : cands) {
+        const double sc = double(absorb(rem, Q | add).size()) / __builtin_popcountll(add);
+        if (sc > best_score) { best_score = sc; best = add; }
+      }
+      Q |= best;
+    }
+    std::vector<int> taken = absorb(rem, Q);
+    if (taken.empty()) return {};  // a gate wider than k
+    out.push_back(taken);
+    std::vector<int> keep;
+    size_t ti = 0;
+    for (int i : rem) {
+      if (ti < taken.size() && taken[ti] == i) { ++ti; continue; }
+      keep.push_back(i);
+    }
+    rem.swap(keep);
+  }
+  return out;
+}
+
+template <typename R, int KT>
+void launch_part_bf(void* amp, const unsigned char* dev, size_t steps_off, size_t frag_off, int nsteps, int grid,
+                    hipStream_t st, uint64_t ntiles, uint64_t outer, int* rc) {
+  const size_t lds = size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);
+  auto kern = &k_part_bf<R, KT>;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)std::max<size_t>(lds, 1)) != hipSuccess) {
+    *rc = HSV_ERR_DEVICE;
+    return;
+  }
+  hipLaunchKernelGGL(kern, grid, kPartBfThreads, lds, st, static_cast<typename VecT<R>::type*>(amp),
+                     reinterpret_cast<const PartBfTables*>(dev),
+                     reinterpret_cast<const PartBfStep*>(dev + steps_off),
+                     reinterpret_cast<const bf8*>(dev + frag_off), nsteps, ntiles, outer);
+}
+
+// A part of 8..13 qubits on the BF16-split path (see k_part_bf).
+int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
+                  const std::vector<Lowered>& gates, uint64_t pmask, std::string* err) {
+  const int K = __builtin_popcountll(pmask);
+  const int KT = K <= 12 ? 12 : 13;
+  if (K > 13) {
+    *err = "the bf16 fused path takes parts of at most 13 qubits";
+    return HSV_ERR_LIMIT;
+  }
+  if (n < KT) {
+    *err = "a wide bf16 fused part needs a state of at least " + std::to_string(KT) + " qubits";
+    return HSV_ERR_LIMIT;
+  }
+  uint64_t tmask = pmask;
+  for (int b = 0; b < n && __builtin_popcountll(tmask) < KT; ++b) tmask |= 1ull << b;
+  int slot[13];
+  for (int b = 0, j = 0; b < 64 && j < KT; ++b)
+    if (tmask >> b & 1) slot[j++] = b;
+  auto inner = [&](int q) { return __builtin_popcountll(tmask & ((1ull << q) - 1)); };
+  std::vector<uint64_t> qm;
+  std::vector<const Lowered*> gl;
+  for (const Lowered& L : gates) {
+    if (L.kind == K_IDENT) continue;
+    uint64_t m = 0;
+    for (int i = 0; i < L.nt; ++i) m |= 1ull << inner(L.t[i]);
+    for (int i = 0; i < L.nc; ++i) m |= 1ull << inner(L.c[i]);
+    qm.push_back(m);
+    gl.push_back(&L);
+  }
+  if (qm.empty()) return HSV_OK;  // identities only
+  std::vector<std::vector<int>> groups = closure_groups(qm, 5);
+  if (groups.empty()) {
+    *err = "a gate acts on more than 5 qubits: the wide bf16 fused path cannot take it";
+    return HSV_ERR_LIMIT;
+  }
+  const int ns = (int)groups.size();
+  std::vector<PartBfStep> steps(ns);
+  std::vector<uint16_t> frags(size_t(ns) * 2 * 4 * 64 * 8);
+  for (int s = 0; s < ns; ++s) {
+    uint64_t gm = 0;
+    std::vector<Lowered> sg;
+    for (int i : groups[s]) { gm |= qm[i]; sg.push_back(*gl[i]); }
+    for (int b = 0; b < KT && __builtin_popcountll(gm) < 5; ++b) gm |= 1ull << b;  // pad: U (x) I
+    int rbit[5], cbit[8];
+    std::vector<int> qubits;
+    for (int b = 0, r = 0, c = 0; b < KT; ++b) {
+      if (gm >> b & 1) { rbit[r++] = b; qubits.push_back(slot[b]); }
+      else cbit[c++] = b;
+    }
+    std::vector<cd> U;
+    build_fused(sg, qubits, &U);
+    const double uerr = unitarity_error(U, 32);
+    if (!(uerr <= 1e-10)) {
+      *err = "fused step matrix is not unitary (max |U^H U - I| = " + std::to_string(uerr) + ")";
+      return HSV_ERR_UNITARY;
+    }
+    PartBfStep& S = steps[s];
+    std::memset(&S, 0, sizeof(S));
+    for (int r = 0; r < 32; ++r)
+      for (int b = 0; b < 5; ++b)
+        if (r >> b & 1) S.rt[r] |= (uint16_t)(1u << rbit[b]);
+    const int ncb = KT - 5;
+    for (int v = 0; v < 16; ++v)
+      for (int j = 0; j < 4; ++j) {
+        if (!(v >> j & 1)) continue;
+        if (j < ncb) S.ctl[v] |= (uint16_t)(1u << cbit[j]);
+        if (4 + j < ncb) S.cth[v] |= (uint16_t)(1u << cbit[4 + j]);
+      }
+    for (int rb = 0; rb < 2; ++rb)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int e = 0; e < 8; ++e) {
+          const int mm = lane % 16;
+          const int row = 16 * rb + 4 * (mm % 4) + mm / 4;
+          const int col = lane / 16 + 4 * e;
+          const cd u = U[size_t(row) * 32 + col];
+          const uint16_t rh = bf16_rne((float)u.real()), ih = bf16_rne((float)u.imag());
+          const uint16_t rl = bf16_rne((float)(u.real() - bf16_value(rh)));
+          const uint16_t il = bf16_rne((float)(u.imag() - bf16_value(ih)));
+          const size_t f = size_t((s * 2 + rb) * 4) * 64 + lane;
+          frags[(f + 0 * 64) * 8 + e] = rh;
+          frags[(f + 1 * 64) * 8 + e] = rl;
+          frags[(f + 2 * 64) * 8 + e] = ih;
+          frags[(f + 3 * 64) * 8 + e] = il;
+        }
+  }
+  PartBfTables T;
+  std::memset(&T, 0, sizeof(T));
+  for (int v = 0; v < 128; ++v)
+    for (int j = 0; j < 7; ++j) {
+      if (!(v >> j & 1)) continue;
+      if (v < 64 && j < 6) T.alo[v] += 1ull << slot[j];
+      if (6 + j < KT) T.ahi[v] += 1ull << slot[6 + j];
+    }
+  // LDS: the tile (FP32) + step tables + tile tables statically, 8 KB per step
+  const size_t static_lds = (size_t(1) << KT) * 8 + sizeof(PartBfStep) * kPartBfMaxSteps + 192 * 8;
+  const int max_steps = (int)std::min<size_t>(kPartBfMaxSteps, (160 * 1024 - static_lds) / (2 * 4 * 64 * 16));
+  const size_t steps_off = sizeof(PartBfTables);
+  const size_t frag_off = (steps_off + sizeof(PartBfStep) * ns + 255) & ~size_t(255);
+  const size_t bytes = frag_off + frags.size() * 2;
+  std::vector<unsigned char> host(bytes);
+  std::memcpy(host.data(), &T, sizeof(T));
+  std::memcpy(host.data() + steps_off, steps.data(), sizeof(PartBfStep) * ns);
+  std::memcpy(host.data() + frag_off, frags.data(), frags.size() * 2);
+  if (int rc = ensure_scratch(scratch, bytes, stream, err)) return rc;
+  if (hipMemcpyAsync(scratch->ptr, host.data(), bytes, hipMemcpyHostToDevice, stream) != hipSuccess) {
+    *err = "fused upload failed";
+    return HSV_ERR_DEVICE;
+  }
+  const uint64_t all = (n == 64) ? ~0ull : ((1ull << n) - 1);
+  const uint64_t outer = all & ~tmask;
+  const uint64_t ntiles = 1ull << (n - KT);
+  const int grid = (int)std::min<uint64_t>(ntiles, uint64_t(num_cu) * 2);
+  const auto* dev = static_cast<const unsigned char*>(scratch->ptr);
+  const bool c128 = dtype == HSV_C128;
+  int rc = HSV_OK;
+  // more steps than fit in LDS at once: one pass per chunk of steps
+  for (int s0 = 0; s0 < ns && rc == HSV_OK; s0 += max_steps) {
+    const int cnt = std::min(max_steps, ns - s0);
+    const size_t so = steps_off + sizeof(PartBfStep) * s0;
+    const size_t fo = frag_off + size_t(s0) * 2 * 4 * 64 * 16;
+    if (KT == 12) c128 ? launch_part_bf<double, 12>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, &rc)
+                       : launch_part_bf<float, 12>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, &rc);
+    else c128 ? launch_part_bf<double, 13>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, &rc)
+              : launch_part_bf<float, 13>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, &rc);
+  }
+  if (rc) { *err = "wide bf16 fused kernel attribute setup failed"; return rc; }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    *err = std::string("fused launch: ") + hipGetErrorString(e);
+    return HSV_ERR_DEVICE;
+  }
+  return HSV_OK;
+}
+
+// variant: 0 / 1 = column-block kernel (1: swizzled row order), 2 = tile form
+// (needs n >= 12; falls back to 1 below that).
 int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
-                   const std::vector<int>& Q, const std::vector<cd>& Um, int swizzle, std::string* err) {
+                   const std::vector<int>& Q, const std::vector<cd>& Um, int variant, std::string* err) {
   const int K = (int)Q.size();
   const int D = 1 << K;
   if (K < 5 || K > 7) {
     *err = "the bf16 fused path takes parts of 5..7 qubits";
     return HSV_ERR_LIMIT;
   }
+  const bool tile = variant == 2 && n >= 12;
+  const int swizzle = variant >= 1 ? 1 : 0;
   const int RB = D / 16, KS = D / 32;
   const size_t frag_bytes = size_t(8) * D * D;
-  std::vector<unsigned char> host(frag_bytes + size_t(D) * 8 + 16 * 8);
+  std::vector<unsigned char> host(frag_bytes + std::max(size_t(D) * 8 + 16 * 8, sizeof(BfTileTables)));
   uint16_t* hf = reinterpret_cast<uint16_t*>(host.data());
   for (int rb = 0; rb < RB; ++rb)
     for (int ks = 0; ks < KS; ++ks)
@@ -397,18 +908,47 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
         }
   uint64_t qmask = 0;
   for (int q : Q) qmask |= 1ull << q;
+  const uint64_t all = (n == 64) ? ~0ull : ((1ull << n) - 1);
+  if (tile) {
+    uint64_t cmask = 0;
+    for (int b = 0, got = 0; b < n && got < 12 - K; ++b)
+      if (!(qmask >> b & 1)) { cmask |= 1ull << b; ++got; }
+    build_tile_tables(qmask, cmask, reinterpret_cast<BfTileTables*>(host.data() + frag_bytes));
+    const size_t bytes = frag_bytes + sizeof(BfTileTables);
+    if (int rc = ensure_scratch(scratch, bytes, stream, err)) return rc;
+    if (hipMemcpyAsync(scratch->ptr, host.data(), bytes, hipMemcpyHostToDevice, stream) != hipSuccess) {
+      *err = "fused upload failed";
+      return HSV_ERR_DEVICE;
+    }
+    const uint64_t ntiles = 1ull << (n - 12);
+    const uint64_t outer = all & ~qmask & ~cmask;
+    const int grid = (int)std::min<uint64_t>(ntiles, uint64_t(num_cu) * 2);
+    const bool c128 = dtype == HSV_C128;
+    if (K == 5) c128 ? launch_bft<double, 5>(amp, scratch->ptr, grid, stream, ntiles, outer)
+                     : launch_bft<float, 5>(amp, scratch->ptr, grid, stream, ntiles, outer);
+    else if (K == 6) c128 ? launch_bft<double, 6>(amp, scratch->ptr, grid, stream, ntiles, outer)
+                          : launch_bft<float, 6>(amp, scratch->ptr, grid, stream, ntiles, outer);
+    else c128 ? launch_bft<double, 7>(amp, scratch->ptr, grid, stream, ntiles, outer)
+              : launch_bft<float, 7>(amp, scratch->ptr, grid, stream, ntiles, outer);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      *err = std::string("fused launch: ") + hipGetErrorString(e);
+      return HSV_ERR_DEVICE;
+    }
+    return HSV_OK;
+  }
   uint64_t tmask = 0;
   for (int b = 0, got = 0; b < n && got < 4; ++b)
     if (!(qmask >> b & 1)) { tmask |= 1ull << b; ++got; }
   uint64_t* hq = reinterpret_cast<uint64_t*>(host.data() + frag_bytes);
   for (int r = 0; r < D; ++r) hq[r] = pdep_host((uint64_t)r, qmask);
   for (int c = 0; c < 16; ++c) hq[D + c] = pdep_host((uint64_t)c, tmask);
-  if (int rc = ensure_scratch(scratch, host.size(), stream, err)) return rc;
-  if (hipMemcpyAsync(scratch->ptr, host.data(), host.size(), hipMemcpyHostToDevice, stream) != hipSuccess) {
+  const size_t bytes = frag_bytes + size_t(D) * 8 + 16 * 8;
+  if (int rc = ensure_scratch(scratch, bytes, stream, err)) return rc;
+  if (hipMemcpyAsync(scratch->ptr, host.data(), bytes, hipMemcpyHostToDevice, stream) != hipSuccess) {
     *err = "fused upload failed";
     return HSV_ERR_DEVICE;
   }
-  const uint64_t all = (n == 64) ? ~0ull : ((1ull << n) - 1);
   BfArgs f{};
   f.outer_mask = all & ~qmask & ~tmask;
   f.nblocks = 1ull << (n - K - 4);
@@ -441,6 +981,16 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
   if (mode != HSV_FUSED_F64 && mode != HSV_FUSED_F32SPLIT && mode != HSV_FUSED_BF16SPLIT) {
     *err = "unknown fused mode";
     return HSV_ERR_ARG;
+  }
+  if (mode == HSV_FUSED_BF16SPLIT) {
+    uint64_t pmask = 0;
+    for (const Lowered& L : gates) {
+      if (L.kind == K_IDENT) continue;
+      for (int i = 0; i < L.nt; ++i) pmask |= 1ull << L.t[i];
+      for (int i = 0; i < L.nc; ++i) pmask |= 1ull << L.c[i];
+    }
+    if (__builtin_popcountll(pmask) > 7)
+      return part_apply_bf(amp, n, dtype, stream, num_cu, scratch, gates, pmask, err);
   }
   std::vector<int> Q;
   std::vector<cd> Um;

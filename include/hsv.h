@@ -164,9 +164,12 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *   "io_bits"     low part bits the register kernel keeps on the lanes in
  *                 its first and last layouts, for coalesced loads and
  *                 stores (0..6; 0 = off, the default);
- *   "fused_swizzle" row order of the BF16-split fused kernel's loads and
- *                 stores (1, the default: the 4 lanes of a column cover 4
- *                 consecutive rows per instruction; 0: 8 rows per lane). */
+ *   "fused_variant" form of the BF16-split fused kernel: 2 (the default,
+ *                 states of >= 12 qubits) = 4096-amplitude tiles loaded and
+ *                 stored coalesced through LDS, U rows in registers;
+ *                 1 = per-wave column blocks streamed into MFMA operands,
+ *                 U in LDS, 4 consecutive rows per column and instruction;
+ *                 0 = the same with 8 rows per lane. */
 int hsv_tune(hsv_state* s, const char* key, int value);
 
 /* ---- Batched QAOA MaxCut (not in the reference) ------------------------

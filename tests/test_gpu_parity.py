@@ -148,6 +148,33 @@ def test_fused_bf16split_infidelity(orc, k, seed, n, dtype):
     assert np.abs(got - want).max() <= 1e-6
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("k,n,ngates,seed", [(8, 14, 40, 1), (10, 16, 60, 2), (12, 16, 90, 3), (12, 20, 400, 4),
+                                             (13, 18, 120, 5), (13, 13, 60, 6)])
+def test_fused_bf16split_wide_parts(orc, k, n, ngates, seed, dtype):
+    """Parts of 8..13 qubits on the BF16-split path: split into fused steps
+    of <= 5 qubits applied to a 12- or 13-qubit tile in LDS (the 400-gate
+    part needs more steps than fit at once and runs in several passes);
+    infidelity and max error within 1e-6 of the FP64 oracle."""
+    rng = np.random.default_rng(900 + seed)
+    parts = random_parts(rng, n, 3, k, ngates // 3, names=UNITARY_ONE + UNITARY_TWO, max_controls=2)
+    a = rand_state(n, seed)
+    want = a.copy()
+    for p in parts:
+        orc.apply_gates(want, p)
+    got = gpu_run(n, a, lambda s: [s.apply_fused(p, "bf16split") for p in parts] + [s.renormalize()], dtype)
+    infid = 1 - abs(np.vdot(want, got)) ** 2
+    assert infid <= 1e-6
+    assert np.abs(got - want).max() <= 1e-6
+
+
+def test_fused_bf16split_wide_rejects_six_qubit_gate():
+    g = Gate("SWAP", (0, 1), (2, 3, 4, 5))   # 6 qubits: wider than a 5-qubit step
+    with capi.State(14) as s:
+        with pytest.raises(capi.HsvError):
+            s.apply_fused([g] + [Gate("H", (q,)) for q in range(6, 10)], "bf16split")
+
+
 def test_fused_bf16split_limits():
     with capi.State(8) as s:   # 5-qubit part leaves 3 qubits outside: below the 4 the path needs
         with pytest.raises(capi.HsvError):
