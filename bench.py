@@ -190,6 +190,8 @@ def main() -> int:
     ap.add_argument("--partitioner", default="best", choices=["best", "greedy", "dfs", "closure"],
                     help="acyclic partitioner (best: the one with fewer parts)")
     ap.add_argument("--pergate-layers", type=int, default=1, help="layers of the per-gate timing sample")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="hsv_tune setting for the timed state (repeatable), e.g. part_tiles_per_wg=2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-full-verify", action="store_true",
@@ -199,7 +201,7 @@ def main() -> int:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU layout is described "
               f"in DESIGN.md and not run here.", file=sys.stderr)
         return 2
-    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 12, "pergate": 1, "permute": 3}[args.mode]
+    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 13, "pergate": 1, "permute": 3}[args.mode]
     n, depth = args.n or (32 if args.mode == "permute" else 33), args.depth
     gates = random_layered(n, depth, seed=0)
     partitioner_used = None
@@ -213,7 +215,10 @@ def main() -> int:
     total_upd = sum(gate_amplitudes(g, n) for g in gates)
     log(f"[bench] n={n} depth={depth} gates={len(gates)} mode={args.mode} k={k} parts={len(parts)}")
 
+    tune = [(kv.split("=")[0], int(kv.split("=")[1])) for kv in args.tune]
     with capi.State(n) as s:
+        for key, value in tune:
+            s.tune(key, value)
         for w in range(args.warmup):
             run_path(s, args.mode, gates, parts)
             s.synchronize()
@@ -328,7 +333,8 @@ def main() -> int:
         "data": "synthetic: seeded random layered circuit (U3 angles ~ U[0,2pi), CZ on random matchings), |0..0> input",
         "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, complex128 state "
                                 f"({2 ** n * 16 / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
-                    "mode": args.mode, "k": k, "passes": passes, "partitioner": partitioner_used}
+                    "mode": args.mode, "k": k, "passes": passes, "partitioner": partitioner_used,
+                    **({"tune": dict(tune)} if tune else {})}
                    if args.mode != "permute" else
                    {"workload": f"exchange permutation of a {n}-qubit complex128 state ({2 ** n * 16 / 1e9:.1f} GB): "
                                 f"qubits 9..{8 + k} to the top slots", "n": n, "mode": "permute", "k": k,

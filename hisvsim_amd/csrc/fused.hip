@@ -159,6 +159,28 @@ __global__ __launch_bounds__(4 * (1 << K)) void k_fused(typename VecT<R>::type* 
 // D[4*(l/16) + i][l%16].
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
+// Non-temporal 2-element loads and stores of an amplitude (one dwordx4 or
+// dwordx2 access with the streaming hint): the tile kernels touch each
+// amplitude once per pass.
+template <typename R> struct Pair2 { typedef R type __attribute__((ext_vector_type(2))); };
+template <typename R>
+__device__ __forceinline__ typename VecT<R>::type ld_stream(const typename VecT<R>::type* p) {
+  using E = typename Pair2<R>::type;
+  const E v = __builtin_nontemporal_load(reinterpret_cast<const E*>(p));
+  typename VecT<R>::type o;
+  o.x = v.x;
+  o.y = v.y;
+  return o;
+}
+template <typename R>
+__device__ __forceinline__ void st_stream(typename VecT<R>::type* p, typename VecT<R>::type x) {
+  using E = typename Pair2<R>::type;
+  E v;
+  v.x = x.x;
+  v.y = x.y;
+  __builtin_nontemporal_store(v, reinterpret_cast<E*>(p));
+}
+
 struct BfArgs {
   uint64_t nblocks;     // column blocks of 16 columns
   uint64_t outer_mask;  // outer bits above the 4 column bits
@@ -282,7 +304,7 @@ template <typename R, int K>
 __global__ __launch_bounds__(kBfTileThreads) void k_fused_bft(typename VecT<R>::type* __restrict__ a,
                                                             const bf8* __restrict__ ufrag,
                                                             const BfTileTables* __restrict__ tabs,
-                                                            uint64_t ntiles, uint64_t outer_mask) {
+                                                            uint64_t ntiles, uint64_t outer_mask, int tiles_per_wg) {
   constexpr int D = 1 << K;
   constexpr int RB = D / 16, KS = D / 32;
   constexpr int C = 4096 / D, NCB = C / 16;
@@ -310,18 +332,20 @@ __global__ __launch_bounds__(kBfTileThreads) void k_fused_bft(typename VecT<R>::
   __syncthreads();
   __bf16* xb = reinterpret_cast<__bf16*>(xs);
   typedef float v4f_t __attribute__((ext_vector_type(4)));
-  V raw[PER];
-  uint64_t tile = blockIdx.x;
-  if (tile < ntiles) {
+  // consecutive tiles per workgroup, no prefetch (the other workgroups on
+  // the CU overlap this one's compute); the U fragments above are loaded
+  // once per workgroup
+  for (int it = 0; it < tiles_per_wg; ++it) {
+    const uint64_t tile = uint64_t(blockIdx.x) * tiles_per_wg + it;
+    if (tile >= ntiles) break;  // uniform across the workgroup
     const uint64_t base = pdep64(tile, outer_mask);
+    V raw[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int t = threadIdx.x + kBfTileThreads * j;
-      raw[j] = a[base + tb.alo[t & 63] + tb.ahi[t >> 6]];
+      raw[j] = ld_stream<R>(&a[base + tb.alo[t & 63] + tb.ahi[t >> 6]]);
     }
-  }
-  for (; tile < ntiles; tile += gridDim.x) {
-    const uint64_t base = pdep64(tile, outer_mask);
+    __syncthreads();  // the previous tile's stores have read ys (and the xs reads are done)
     // split and scatter this thread's elements into the B fragments:
     // row r = 32 ks + g + 4 e goes to lane 16 g + (c % 16), element e
 #pragma unroll
@@ -337,15 +361,6 @@ __global__ __launch_bounds__(kBfTileThreads) void k_fused_bft(typename VecT<R>::
       xb[(frag + 1 * 64) * 8 + e] = (__bf16)(xr - (float)hr);
       xb[(frag + 2 * 64) * 8 + e] = hi;
       xb[(frag + 3 * 64) * 8 + e] = (__bf16)(xi - (float)hi);
-    }
-    const uint64_t tn = tile + gridDim.x;
-    if (tn < ntiles) {
-      const uint64_t nbase = pdep64(tn, outer_mask);
-#pragma unroll
-      for (int j = 0; j < PER; ++j) {
-        const int t = threadIdx.x + kBfTileThreads * j;
-        raw[j] = a[nbase + tb.alo[t & 63] + tb.ahi[t >> 6]];
-      }
     }
     __syncthreads();
 #pragma unroll
@@ -385,7 +400,7 @@ __global__ __launch_bounds__(kBfTileThreads) void k_fused_bft(typename VecT<R>::
       V o;
       o.x = (R)y.x;
       o.y = (R)y.y;
-      a[base + tb.alo[t & 63] + tb.ahi[t >> 6]] = o;
+      st_stream<R>(&a[base + tb.alo[t & 63] + tb.ahi[t >> 6]], o);
     }
   }
 }
@@ -401,81 +416,133 @@ __global__ __launch_bounds__(kBfTileThreads) void k_fused_bft(typename VecT<R>::
 // column blocks of 16 and updates them in place; a barrier between steps),
 // and stores the tile back coalesced. One HBM pass per part, the gate work
 // on the matrix cores.
+//
+// LDS bank swizzle: tile element t lives at ys[t ^ h(t >> 4)], h linear over
+// GF(2) (swz[] tabulates it; tile bit 4 feeds only bits 0-3, so the map is
+// invertible), chosen on the host per part so that the 16
+// lanes of a quarter-wave (16 columns of one row) and the two rows of a
+// half-wave fall in distinct banks for every step. Because the map is
+// linear, a step's row and column offsets are tabulated already swizzled and
+// combined with XOR. Without it, a step on the low tile bits reads its
+// columns at a stride of 32+ elements: 16-way bank conflicts.
 struct PartBfStep {
-  uint16_t rt[32];              // tile index bits of step row r
-  uint16_t ctl[16], cth[16];    // of step column c: ctl[c & 15] | cth[c >> 4]
+  uint16_t rt[32];              // swizzled LDS offset of step row r
+  uint16_t ctl[16], cth[16];    // of step column c: ctl[c & 15] ^ cth[c >> 4]
 };
 struct PartBfTables {
   uint64_t alo[64], ahi[128];   // address offset of tile index bits 0-5 / 6-12
+  uint8_t swz[512];             // h(t >> 4): XOR into the low 5 bits of t
 };
-constexpr int kPartBfThreads = 512;
+// 512 threads: 8 amplitudes per thread for 12-qubit tiles, 16 for 13.
+// 12-qubit tiles keep every step's U fragments in LDS; 13-qubit tiles (64 KB
+// of FP32 in LDS) stream them through one 8 KB buffer, each thread holding
+// its 16-byte share of the next step in a register, so two workgroups fit a
+// CU and one computes while the other loads or stores.
+constexpr int part_bf_threads(int) { return 512; }
 constexpr int kPartBfMaxSteps = 12;
 
 template <typename R, int KT>
-__global__ __launch_bounds__(kPartBfThreads) void k_part_bf(typename VecT<R>::type* __restrict__ a,
+// two workgroups of 8 waves per CU: at most 128 VGPRs
+__global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_per_eu(4))) void k_part_bf(typename VecT<R>::type* __restrict__ a,
                                                           const PartBfTables* __restrict__ tabs,
                                                           const PartBfStep* __restrict__ steps_g,
                                                           const bf8* __restrict__ ufrag, int nsteps,
-                                                          uint64_t ntiles, uint64_t outer_mask) {
+                                                          uint64_t ntiles, uint64_t outer_mask, int tiles_per_wg) {
   constexpr int T = 1 << KT;
   constexpr int NCB = (1 << (KT - 5)) / 16;   // column blocks of a step
-  constexpr int W = kPartBfThreads / 64;
+  constexpr int NTH = part_bf_threads(KT);
+  constexpr int W = NTH / 64;
   constexpr int CBW = NCB / W;                // column blocks per wave
-  constexpr int PER = T / kPartBfThreads;
+  constexpr int PER = T / NTH;
+  constexpr bool STREAM_U = KT >= 13;
+  constexpr int STEP_FRAGS = 2 * 4 * 64;      // bf8 per step: [rb][set][lane]
+  static_assert(CBW >= 1 && NCB % W == 0 && PER * NTH == T, "tile shape");
+  static_assert(!STREAM_U || STEP_FRAGS == NTH, "one fragment per thread per step");
   using V = typename VecT<R>::type;
   typedef float v4f_t __attribute__((ext_vector_type(4)));
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  bf8* su = reinterpret_cast<bf8*>(smem_raw);                           // [step][rb][set][lane]
+  __shared__ bf8 su_one[STREAM_U ? STEP_FRAGS : 1];
+  bf8* su = STREAM_U ? su_one : reinterpret_cast<bf8*>(smem_raw);     // [step][rb][set][lane]
   __shared__ float2 ys[T];
   __shared__ PartBfStep st[kPartBfMaxSteps];
   __shared__ uint64_t alo[64], ahi[128];
-  for (int i = threadIdx.x; i < nsteps * 2 * 4 * 64; i += kPartBfThreads) su[i] = ufrag[i];
+  __shared__ uint8_t swz[T >> 4];
+  if constexpr (!STREAM_U)
+    for (int i = threadIdx.x; i < nsteps * STEP_FRAGS; i += NTH) su[i] = ufrag[i];
   {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(steps_g);
     uint32_t* dst = reinterpret_cast<uint32_t*>(st);
-    for (int i = threadIdx.x; i < nsteps * (int)(sizeof(PartBfStep) / 4); i += kPartBfThreads) dst[i] = src[i];
+    for (int i = threadIdx.x; i < nsteps * (int)(sizeof(PartBfStep) / 4); i += NTH) dst[i] = src[i];
   }
-  for (int i = threadIdx.x; i < 64; i += kPartBfThreads) alo[i] = tabs->alo[i];
-  for (int i = threadIdx.x; i < 128; i += kPartBfThreads) ahi[i] = tabs->ahi[i];
-  __syncthreads();
+  for (int i = threadIdx.x; i < 64; i += NTH) alo[i] = tabs->alo[i];
+  for (int i = threadIdx.x; i < 128; i += NTH) ahi[i] = tabs->ahi[i];
+  for (int i = threadIdx.x; i < (T >> 4); i += NTH) swz[i] = tabs->swz[i];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane >> 4, cc = lane & 15;
+  // A few consecutive tiles per workgroup (tiles_per_wg; the grid covers
+  // the rest). With the fragments resident, the next tile's loads are issued
+  // before this tile's steps and overlap them. (With streamed fragments the
+  // per-step fragment loads would wait behind such a prefetch, so there the
+  // other workgroup on the CU provides the overlap.)
+  const uint64_t t0 = uint64_t(blockIdx.x) * tiles_per_wg;
+  if (t0 >= ntiles) return;  // uniform across the workgroup
+  const int nt = (int)min(uint64_t(tiles_per_wg), ntiles - t0);
+  bf8 fr;  // streamed fragments: this thread's share of the next step
+  if constexpr (STREAM_U) fr = ufrag[threadIdx.x];
+  __syncthreads();  // tables ready
   V raw[PER];
-  uint64_t tile = blockIdx.x;
-  if (tile < ntiles) {
-    const uint64_t base = pdep64(tile, outer_mask);
+  if constexpr (!STREAM_U) {
+    const uint64_t base = pdep64(t0, outer_mask);
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const int t = threadIdx.x + kPartBfThreads * j;
-      raw[j] = a[base + alo[t & 63] + ahi[t >> 6]];
+      const int t = threadIdx.x + NTH * j;
+      raw[j] = ld_stream<R>(&a[base + alo[t & 63] + ahi[t >> 6]]);
     }
   }
-  for (; tile < ntiles; tile += gridDim.x) {
-    const uint64_t base = pdep64(tile, outer_mask);
-#pragma unroll
-    for (int j = 0; j < PER; ++j) ys[threadIdx.x + kPartBfThreads * j] = make_float2((float)raw[j].x, (float)raw[j].y);
-    const uint64_t tn = tile + gridDim.x;
-    if (tn < ntiles) {
-      const uint64_t nbase = pdep64(tn, outer_mask);
+  for (int it = 0; it < nt; ++it) {
+    const uint64_t base = pdep64(t0 + it, outer_mask);
+    if constexpr (STREAM_U) {
 #pragma unroll
       for (int j = 0; j < PER; ++j) {
-        const int t = threadIdx.x + kPartBfThreads * j;
-        raw[j] = a[nbase + alo[t & 63] + ahi[t >> 6]];
+        const int t = threadIdx.x + NTH * j;
+        raw[j] = ld_stream<R>(&a[base + alo[t & 63] + ahi[t >> 6]]);
+      }
+    }
+    if (it > 0) __syncthreads();  // the previous tile's stores have read ys
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int t = threadIdx.x + NTH * j;
+      ys[t ^ swz[t >> 4]] = make_float2((float)raw[j].x, (float)raw[j].y);
+    }
+    if (!STREAM_U && it + 1 < nt) {
+      const uint64_t nbase = pdep64(t0 + it + 1, outer_mask);
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        const int t = threadIdx.x + NTH * j;
+        raw[j] = ld_stream<R>(&a[nbase + alo[t & 63] + ahi[t >> 6]]);
       }
     }
     for (int s = 0; s < nsteps; ++s) {
       __syncthreads();  // the tile (or the previous step's results) is in LDS
       const PartBfStep& S = st[s];
-      const bf8* As = su + s * 2 * 4 * 64 + lane;
+      const bf8* As;
+      if constexpr (STREAM_U) {
+        su_one[threadIdx.x] = fr;  // every wave is past the previous step's reads
+        fr = ufrag[(s + 1 < nsteps ? s + 1 : 0) * STEP_FRAGS + threadIdx.x];
+        __syncthreads();
+        As = su_one + lane;
+      } else {
+        As = su + s * STEP_FRAGS + lane;
+      }
 #pragma unroll
       for (int w = 0; w < CBW; ++w) {
         const int col = 16 * (wave * CBW + w) + cc;
-        const int tc = S.ctl[col & 15] | S.cth[col >> 4];
+        const int tc = S.ctl[col & 15] ^ S.cth[col >> 4];
         bf8 xrh, xrl, xih, xil;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          const float2 v = ys[S.rt[g + 4 * e] | tc];
+          const float2 v = ys[S.rt[g + 4 * e] ^ tc];
           const __bf16 hr = (__bf16)v.x, hi = (__bf16)v.y;
           xrh[e] = hr;
           xrl[e] = (__bf16)(v.x - (float)hr);
@@ -502,19 +569,19 @@ __global__ __launch_bounds__(kPartBfThreads) void k_part_bf(typename VecT<R>::ty
           // this wave read every row of its column block above: write in place
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            ys[S.rt[16 * rb + 4 * i + g] | tc] = make_float2(accR[i] - accT[i], accI[i]);
+            ys[S.rt[16 * rb + 4 * i + g] ^ tc] = make_float2(accR[i] - accT[i], accI[i]);
         }
       }
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const int t = threadIdx.x + kPartBfThreads * j;
-      const float2 y = ys[t];
+      const int t = threadIdx.x + NTH * j;
+      const float2 y = ys[t ^ swz[t >> 4]];
       V o;
       o.x = (R)y.x;
       o.y = (R)y.y;
-      a[base + alo[t & 63] + ahi[t >> 6]] = o;
+      st_stream<R>(&a[base + alo[t & 63] + ahi[t >> 6]], o);
     }
   }
 }
@@ -626,12 +693,12 @@ void launch_bf(void* amp, const void* scratch, int grid, hipStream_t st, const B
 // The BF16-split path: U as BF16 hi/lo fragments in the MFMA A-operand order,
 // columns in blocks of 16 along the lowest 4 slots outside the part.
 template <typename R, int K>
-void launch_bft(void* amp, const void* scratch, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer) {
+void launch_bft(void* amp, const void* scratch, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer, int tpw) {
   const size_t frag_bytes = size_t(8) << (2 * K);
   const auto* U = static_cast<const bf8*>(scratch);
   const auto* T = reinterpret_cast<const BfTileTables*>(static_cast<const unsigned char*>(scratch) + frag_bytes);
   hipLaunchKernelGGL((k_fused_bft<R, K>), grid, kBfTileThreads, 0, st,
-                     static_cast<typename VecT<R>::type*>(amp), U, T, ntiles, outer);
+                     static_cast<typename VecT<R>::type*>(amp), U, T, ntiles, outer, tpw);
 }
 
 // Tables of the tile form: tile index bit j <-> the j-th lowest of the tile's
@@ -723,27 +790,81 @@ std::vector<std::vector<int>> closure_groups(const std::vector<uint64_t>& qm, in
   return out;
 }
 
+// LDS swizzle of k_part_bf. nimg[c] = h applied to tile bit c: 0 for
+// c < 4, a mask of bits 0-3 for c = 4, of bits 0-4 for c >= 5. A step is
+// conflict-free when the low-4-bit images of its 4 lowest column bits are
+// independent (the 16 lanes of a quarter-wave) and, with its lowest row bit,
+// the low-5-bit images are too (the 32 lanes of a half-wave).
+// rank over GF(2) of n <= 5 vectors of 5 bits
+int rank_gf2(uint32_t* v, int n) {
+  int r = 0;
+  for (int bit = 4; bit >= 0; --bit) {
+    int piv = -1;
+    for (int i = r; i < n; ++i)
+      if (v[i] >> bit & 1) { piv = i; break; }
+    if (piv < 0) continue;
+    std::swap(v[r], v[piv]);
+    for (int i = 0; i < n; ++i)
+      if (i != r && (v[i] >> bit & 1)) v[i] ^= v[r];
+    ++r;
+  }
+  return r;
+}
+
+struct PartSwizzleKey { int cbit[4]; int rbit0; };
+
+void choose_part_swizzle(int KT, const std::vector<PartSwizzleKey>& keys, uint32_t nimg[13]) {
+  auto score = [&](const uint32_t* m) {
+    int sc = 0;
+    for (const PartSwizzleKey& k : keys) {
+      auto low5 = [&](int c) { return ((c < 5) ? (1u << c) : 0u) ^ m[c]; };
+      uint32_t a[4], b[5];
+      for (int j = 0; j < 4; ++j) { b[j] = low5(k.cbit[j]); a[j] = b[j] & 15u; }
+      b[4] = low5(k.rbit0);
+      sc += 2 * (rank_gf2(a, 4) == 4) + (rank_gf2(b, 5) == 5);
+    }
+    return sc;
+  };
+  const int best_possible = 3 * (int)keys.size();
+  uint32_t cand[13] = {0};
+  std::memcpy(nimg, cand, sizeof(cand));
+  int best = score(cand);
+  uint64_t rng = 0x9E3779B97F4A7C15ull;
+  for (int it = 0; it < 2000 && best < best_possible; ++it) {
+    for (int c = 0; c < 13; ++c) {
+      rng ^= rng << 13; rng ^= rng >> 7; rng ^= rng << 17;
+      cand[c] = c < 4 ? 0u : (c == 4 ? uint32_t(rng & 15u) : (c < KT ? uint32_t(rng & 31u) : 0u));
+    }
+    const int sc = score(cand);
+    if (sc > best) { best = sc; std::memcpy(nimg, cand, sizeof(cand)); }
+  }
+}
+
 template <typename R, int KT>
 void launch_part_bf(void* amp, const unsigned char* dev, size_t steps_off, size_t frag_off, int nsteps, int grid,
-                    hipStream_t st, uint64_t ntiles, uint64_t outer, int* rc) {
-  const size_t lds = size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);
+                    hipStream_t st, uint64_t ntiles, uint64_t outer, int tpw, int* rc) {
+  const size_t lds = KT >= 13 ? 0 : size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);  // 13: fragments streamed
   auto kern = &k_part_bf<R, KT>;
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)std::max<size_t>(lds, 1)) != hipSuccess) {
     *rc = HSV_ERR_DEVICE;
     return;
   }
-  hipLaunchKernelGGL(kern, grid, kPartBfThreads, lds, st, static_cast<typename VecT<R>::type*>(amp),
+  hipLaunchKernelGGL(kern, grid, part_bf_threads(KT), lds, st, static_cast<typename VecT<R>::type*>(amp),
                      reinterpret_cast<const PartBfTables*>(dev),
                      reinterpret_cast<const PartBfStep*>(dev + steps_off),
-                     reinterpret_cast<const bf8*>(dev + frag_off), nsteps, ntiles, outer);
+                     reinterpret_cast<const bf8*>(dev + frag_off), nsteps, ntiles, outer, tpw);
 }
 
 // A part of 8..13 qubits on the BF16-split path (see k_part_bf).
-int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
-                  const std::vector<Lowered>& gates, uint64_t pmask, std::string* err) {
+int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
+                  const std::vector<Lowered>& gates, uint64_t pmask, int tpw, std::string* err) {
   const int K = __builtin_popcountll(pmask);
-  const int KT = K <= 12 ? 12 : 13;
+  // The tile is the part's qubits padded with the lowest free ones. It should
+  // hold qubits 0-4, so that each wave's loads and stores cover 512-byte
+  // runs: a part that fits 12 qubits only without them takes a 13-qubit tile.
+  const uint64_t low5 = (1ull << std::min(n, 5)) - 1;
+  const int KT = __builtin_popcountll(pmask | low5) <= 12 ? 12 : 13;
   if (K > 13) {
     *err = "the bf16 fused path takes parts of at most 13 qubits";
     return HSV_ERR_LIMIT;
@@ -776,6 +897,7 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, D
   }
   const int ns = (int)groups.size();
   std::vector<PartBfStep> steps(ns);
+  std::vector<PartSwizzleKey> keys(ns);
   std::vector<uint16_t> frags(size_t(ns) * 2 * 4 * 64 * 8);
   for (int s = 0; s < ns; ++s) {
     uint64_t gm = 0;
@@ -797,6 +919,7 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, D
     }
     PartBfStep& S = steps[s];
     std::memset(&S, 0, sizeof(S));
+    keys[s] = PartSwizzleKey{{cbit[0], cbit[1], cbit[2], cbit[3]}, rbit[0]};
     for (int r = 0; r < 32; ++r)
       for (int b = 0; b < 5; ++b)
         if (r >> b & 1) S.rt[r] |= (uint16_t)(1u << rbit[b]);
@@ -824,8 +947,23 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, D
           frags[(f + 3 * 64) * 8 + e] = il;
         }
   }
+  // swizzle the steps' LDS offsets (h is linear: XOR of the bits' images)
+  uint32_t nimg[13];
+  choose_part_swizzle(KT, keys, nimg);
+  auto hmap = [&](uint32_t t) {
+    uint32_t h = 0;
+    for (int c = 4; c < KT; ++c)
+      if (t >> c & 1) h ^= nimg[c];
+    return h;
+  };
+  for (PartBfStep& S : steps) {
+    for (auto& x : S.rt) x = (uint16_t)(x ^ hmap(x));
+    for (auto& x : S.ctl) x = (uint16_t)(x ^ hmap(x));
+    for (auto& x : S.cth) x = (uint16_t)(x ^ hmap(x));
+  }
   PartBfTables T;
   std::memset(&T, 0, sizeof(T));
+  for (int i = 0; i < (1 << (KT - 4)); ++i) T.swz[i] = (uint8_t)hmap(uint32_t(i) << 4);
   for (int v = 0; v < 128; ++v)
     for (int j = 0; j < 7; ++j) {
       if (!(v >> j & 1)) continue;
@@ -833,8 +971,10 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, D
       if (6 + j < KT) T.ahi[v] += 1ull << slot[6 + j];
     }
   // LDS: the tile (FP32) + step tables + tile tables statically, 8 KB per step
-  const size_t static_lds = (size_t(1) << KT) * 8 + sizeof(PartBfStep) * kPartBfMaxSteps + 192 * 8;
-  const int max_steps = (int)std::min<size_t>(kPartBfMaxSteps, (160 * 1024 - static_lds) / (2 * 4 * 64 * 16));
+  const size_t static_lds =
+      (size_t(1) << KT) * 8 + sizeof(PartBfStep) * kPartBfMaxSteps + 192 * 8 + (size_t(1) << (KT - 4));
+  const int max_steps = KT >= 13 ? kPartBfMaxSteps
+                                 : (int)std::min<size_t>(kPartBfMaxSteps, (160 * 1024 - static_lds) / (2 * 4 * 64 * 16));
   const size_t steps_off = sizeof(PartBfTables);
   const size_t frag_off = (steps_off + sizeof(PartBfStep) * ns + 255) & ~size_t(255);
   const size_t bytes = frag_off + frags.size() * 2;
@@ -850,7 +990,8 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, D
   const uint64_t all = (n == 64) ? ~0ull : ((1ull << n) - 1);
   const uint64_t outer = all & ~tmask;
   const uint64_t ntiles = 1ull << (n - KT);
-  const int grid = (int)std::min<uint64_t>(ntiles, uint64_t(num_cu) * 2);
+  if (tpw <= 0) tpw = KT == 12 ? 8 : 1;
+  const int grid = (int)((ntiles + tpw - 1) / tpw);
   const auto* dev = static_cast<const unsigned char*>(scratch->ptr);
   const bool c128 = dtype == HSV_C128;
   int rc = HSV_OK;
@@ -859,10 +1000,10 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, D
     const int cnt = std::min(max_steps, ns - s0);
     const size_t so = steps_off + sizeof(PartBfStep) * s0;
     const size_t fo = frag_off + size_t(s0) * 2 * 4 * 64 * 16;
-    if (KT == 12) c128 ? launch_part_bf<double, 12>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, &rc)
-                       : launch_part_bf<float, 12>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, &rc);
-    else c128 ? launch_part_bf<double, 13>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, &rc)
-              : launch_part_bf<float, 13>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, &rc);
+    if (KT == 12) c128 ? launch_part_bf<double, 12>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
+                       : launch_part_bf<float, 12>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
+    else c128 ? launch_part_bf<double, 13>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
+              : launch_part_bf<float, 13>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
   }
   if (rc) { *err = "wide bf16 fused kernel attribute setup failed"; return rc; }
   const hipError_t e = hipGetLastError();
@@ -876,7 +1017,7 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, D
 // variant: 0 / 1 = column-block kernel (1: swizzled row order), 2 = tile form
 // (needs n >= 12; falls back to 1 below that).
 int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
-                   const std::vector<int>& Q, const std::vector<cd>& Um, int variant, std::string* err) {
+                   const std::vector<int>& Q, const std::vector<cd>& Um, int variant, int tpw, std::string* err) {
   const int K = (int)Q.size();
   const int D = 1 << K;
   if (K < 5 || K > 7) {
@@ -922,14 +1063,14 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
     }
     const uint64_t ntiles = 1ull << (n - 12);
     const uint64_t outer = all & ~qmask & ~cmask;
-    const int grid = (int)std::min<uint64_t>(ntiles, uint64_t(num_cu) * 2);
+    const int grid = (int)((ntiles + tpw - 1) / tpw);
     const bool c128 = dtype == HSV_C128;
-    if (K == 5) c128 ? launch_bft<double, 5>(amp, scratch->ptr, grid, stream, ntiles, outer)
-                     : launch_bft<float, 5>(amp, scratch->ptr, grid, stream, ntiles, outer);
-    else if (K == 6) c128 ? launch_bft<double, 6>(amp, scratch->ptr, grid, stream, ntiles, outer)
-                          : launch_bft<float, 6>(amp, scratch->ptr, grid, stream, ntiles, outer);
-    else c128 ? launch_bft<double, 7>(amp, scratch->ptr, grid, stream, ntiles, outer)
-              : launch_bft<float, 7>(amp, scratch->ptr, grid, stream, ntiles, outer);
+    if (K == 5) c128 ? launch_bft<double, 5>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw)
+                     : launch_bft<float, 5>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw);
+    else if (K == 6) c128 ? launch_bft<double, 6>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw)
+                          : launch_bft<float, 6>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw);
+    else c128 ? launch_bft<double, 7>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw)
+              : launch_bft<float, 7>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       *err = std::string("fused launch: ") + hipGetErrorString(e);
@@ -977,7 +1118,7 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
 }  // namespace
 
 int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
-                const std::vector<Lowered>& gates, int mode, std::string* err, int bf_swizzle) {
+                const std::vector<Lowered>& gates, int mode, std::string* err, const BfTune& bf) {
   if (mode != HSV_FUSED_F64 && mode != HSV_FUSED_F32SPLIT && mode != HSV_FUSED_BF16SPLIT) {
     *err = "unknown fused mode";
     return HSV_ERR_ARG;
@@ -990,7 +1131,7 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
       for (int i = 0; i < L.nc; ++i) pmask |= 1ull << L.c[i];
     }
     if (__builtin_popcountll(pmask) > 7)
-      return part_apply_bf(amp, n, dtype, stream, num_cu, scratch, gates, pmask, err);
+      return part_apply_bf(amp, n, dtype, stream, scratch, gates, pmask, bf.part_tpw, err);
   }
   std::vector<int> Q;
   std::vector<cd> Um;
@@ -1008,7 +1149,7 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
     return HSV_ERR_LIMIT;
   }
   if (mode == HSV_FUSED_BF16SPLIT)
-    return fused_apply_bf(amp, n, dtype, stream, num_cu, scratch, Q, Um, bf_swizzle, err);
+    return fused_apply_bf(amp, n, dtype, stream, num_cu, scratch, Q, Um, bf.variant, std::max(bf.tile_tpw, 1), err);
   int NC = 2048 / (16 * D);
   while (NC > 1 && 4 + __builtin_ctz(NC) > free_bits) NC >>= 1;
   const int C = 16 * NC;

@@ -18,13 +18,22 @@ struct DeviceScratch {
   size_t bytes = 0;
 };
 
+// Launch choices of the BF16-split kernels.
+struct BfTune {
+  int variant = 2;   // parts of 5..7 qubits: 0 / 1 column-block kernel (1: four
+                     // consecutive rows per column in each load), 2 tile kernel
+  int tile_tpw = 8;  // tiles of 4096 amplitudes per workgroup, tile kernel
+  int part_tpw = 0;  // tiles per workgroup, wide-part kernel (8..13 qubits);
+                     // 0: 8 for 12-qubit tiles (the next tile is prefetched
+                     // during the steps), 1 for 13-qubit tiles
+};
+
 // Applies the fused unitary of `gates` to the n-qubit state at `amp`
 // (dtype HSV_C128 or HSV_C64) on `stream`. mode: hsv_fused_mode.
-// `num_cu` sizes the persistent grid. bf_swizzle selects the row order of
-// the BF16-split kernel's loads and stores (1: four consecutive rows per
-// column in each instruction). Returns 0 or a negative hsv_status.
+// `num_cu` sizes the persistent grids of the FP64 / FP32-split kernels.
+// Returns 0 or a negative hsv_status.
 int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
-                const std::vector<Lowered>& gates, int mode, std::string* err, int bf_swizzle = 1);
+                const std::vector<Lowered>& gates, int mode, std::string* err, const BfTune& bf = BfTune());
 
 // The fused unitary itself (for tests and the cost model): qubits of the
 // part (sorted, padded with the lowest free qubits to at least min_qubits)

@@ -72,7 +72,7 @@ struct hsv_state {
   // by default: at 31 qubits it measured no gain (the extra layout change
   // costs what the coalescing saves).
   int io_bits = 0;
-  int fused_variant = 2;             // BF16-split fused kernel form (see hsv_tune)
+  BfTune bf;                         // BF16-split fused kernel form and launch shape (see hsv_tune)
   DeviceScratch fused_scratch;       // fused unitary + offset tables
   size_t elem() const { return dtype == HSV_C128 ? 16 : 8; }
 };
@@ -595,7 +595,7 @@ int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode
   if (int rc = begin_timed(s)) return rc;
   std::string err;
   int rc = fused_apply(s->amp, s->n, s->dtype, s->stream, s->num_cu, &s->fused_scratch, Ls, mode, &err,
-                       s->fused_variant);
+                       s->bf);
   if (rc) return fail(rc, err);
   g_part_count++;
   g_gate_count += num_gates;
@@ -870,7 +870,9 @@ int hsv_tune(hsv_state* s, const char* key, int value) {
   else if (k == "reg_bits" && value >= 0 && value <= 5) s->reg_bits = value;
   else if (k == "low_slots" && value >= 0 && value <= 6) s->low_slots = value;
   else if (k == "io_bits" && value >= 0 && value <= 6) s->io_bits = value;
-  else if (k == "fused_variant" && value >= 0 && value <= 2) s->fused_variant = value;
+  else if (k == "fused_variant" && value >= 0 && value <= 2) s->bf.variant = value;
+  else if (k == "fused_tiles_per_wg" && value >= 1 && value <= 64) s->bf.tile_tpw = value;
+  else if (k == "part_tiles_per_wg" && value >= 0 && value <= 64) s->bf.part_tpw = value;
   else return fail(HSV_ERR_ARG, "unknown tuning key or value out of range: " + k);
   return HSV_OK;
 }

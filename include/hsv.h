@@ -169,7 +169,13 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *                 stored coalesced through LDS, U rows in registers;
  *                 1 = per-wave column blocks streamed into MFMA operands,
  *                 U in LDS, 4 consecutive rows per column and instruction;
- *                 0 = the same with 8 rows per lane. */
+ *                 0 = the same with 8 rows per lane;
+ *   "fused_tiles_per_wg" 4096-amplitude tiles each workgroup of the tile
+ *                 form takes in turn (1..64, default 8);
+ *   "part_tiles_per_wg"  the same for the wide BF16-split part kernel
+ *                 (parts of 8..13 qubits; 0..64; 0, the default, = 8 for
+ *                 12-qubit tiles, whose next tile is prefetched during the
+ *                 steps, and 1 for 13-qubit tiles). */
 int hsv_tune(hsv_state* s, const char* key, int value);
 
 /* ---- Batched QAOA MaxCut (not in the reference) ------------------------

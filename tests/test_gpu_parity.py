@@ -168,6 +168,29 @@ def test_fused_bf16split_wide_parts(orc, k, n, ngates, seed, dtype):
     assert np.abs(got - want).max() <= 1e-6
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_fused_bf16split_wide_part_off_the_low_qubits(orc, dtype):
+    """An 11-qubit part on qubits 2..12: the tile adds qubits 0 and 1 so that
+    it holds qubits 0-4 (a 13-qubit tile), and the steps pad with them."""
+    n = 16
+    rng = np.random.default_rng(77)
+    qs = list(range(2, 13))
+    gates = []
+    for i in range(60):
+        if i % 3 == 2:
+            c, t = rng.choice(qs, 2, replace=False)
+            gates.append(Gate("X", (int(t),), (int(c),)))
+        else:
+            q = int(rng.choice(qs))
+            gates.append(Gate("U3", (q,), (), tuple(float(x) for x in rng.uniform(0, 2 * np.pi, 3))))
+    a = rand_state(n, 5)
+    want = a.copy()
+    orc.apply_gates(want, gates)
+    got = gpu_run(n, a, lambda s: [s.apply_fused(gates, "bf16split"), s.renormalize()], dtype)
+    assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
+    assert np.abs(got - want).max() <= 1e-6
+
+
 def test_fused_bf16split_wide_rejects_six_qubit_gate():
     g = Gate("SWAP", (0, 1), (2, 3, 4, 5))   # 6 qubits: wider than a 5-qubit step
     with capi.State(14) as s:
