@@ -5,11 +5,14 @@ beside it.
 
     python bench.py [--steps K] [--warmup W] [--n 33] [--depth 40]
                     [--mode fusedbf16|part|fused64|fused32|pergate|permute] [--k K]
+                    [--layout chain|fixed] [--tune KEY=VALUE ...]
 
 One step = one full simulation of the circuit from |0...0> (state reset
 included) with the chosen path; the state (2^n amplitudes) stays in HBM. The
-default path is the fused-MFMA one: parts of up to 7 qubits fused into FP64
-unitaries and applied with BF16-split products accumulated in FP32. The
+default path is the fused-MFMA one: parts of up to 13 qubits, each cut into
+steps of 5 qubits fused into FP64 unitaries and applied in one pass with
+BF16-split products accumulated in FP32, the 5 lowest slots moving with the
+parts (two in-place swap passes restore the standard layout at the end). The
 per-gate path is timed on a bounded sample of the same circuit (its first
 layers) and its full time-to-solution is extrapolated per gate. Accuracy of
 each path is checked against the FP64 CPU oracle on the same generator at a
@@ -32,7 +35,7 @@ sys.path.insert(0, ROOT)
 
 from hisvsim_amd import capi  # noqa: E402
 from hisvsim_amd.circuit import gate_amplitudes, random_layered  # noqa: E402
-from hisvsim_amd.partition import closure_parts, dfs_parts, greedy_parts  # noqa: E402
+from hisvsim_amd.partition import chain_parts, closure_parts, dfs_parts, greedy_parts  # noqa: E402
 
 METRIC = "33q circuit time-to-solution + HBM GB/s; fused-MFMA vs per-gate at ≤1e-6 infid"
 HBM_PEAK_GBS = 8000.0
@@ -47,11 +50,16 @@ def log(msg: str) -> None:
 LOW_SLOTS = 5  # qubits the device adds to every LDS-staged part (libhsv default)
 
 
-def make_parts(gates, mode: str, k: int, partitioner: str = "best"):
+def make_parts(gates, mode: str, k: int, partitioner: str = "best", layout: str = "fixed"):
     """Acyclic parts: DAG-aware greedy, depth-first order, qubit-set
     closure, or whichever gives the fewest parts ("best"). For the register
     part kernel, k counts the low qubits the device adds to every part
-    anyway, so the partitioner starts each part with them."""
+    anyway, so the partitioner starts each part with them. With the moving
+    layout (wide BF16-split parts only) the parts come from chain_parts and
+    the low slots follow them. Returns (parts, partitioner, lows or None)."""
+    if mode == "fusedbf16" and k >= 8 and layout == "chain":
+        parts, lows = chain_parts(gates, k, LOW_SLOTS)
+        return parts, "chain", lows
     # the register kernel and the wide (k >= 8) BF16-split kernel load whole
     # parts as tiles, so their parts start with the low qubits for long runs
     always = range(LOW_SLOTS) if mode == "part" or (mode == "fusedbf16" and k >= 8) else ()
@@ -59,8 +67,9 @@ def make_parts(gates, mode: str, k: int, partitioner: str = "best"):
              "dfs": lambda: dfs_parts(gates, k, always=always),
              "closure": lambda: closure_parts(gates, k, always=always)}
     if partitioner != "best":
-        return cands[partitioner](), partitioner
-    return min(((f(), name) for name, f in cands.items()), key=lambda x: len(x[0]))
+        return cands[partitioner](), partitioner, None
+    parts, name = min(((f(), name) for name, f in cands.items()), key=lambda x: len(x[0]))
+    return parts, name, None
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
@@ -99,7 +108,7 @@ def exchange_permutation(n: int, m: int = 3):
     return new_slot
 
 
-def run_path(s: capi.State, mode: str, gates, parts) -> None:
+def run_path(s: capi.State, mode: str, gates, parts, lows=None) -> None:
     if mode == "permute":
         s.permute_qubits(parts)
         return
@@ -111,13 +120,17 @@ def run_path(s: capi.State, mode: str, gates, parts) -> None:
             s.apply_part(p, 0)
     else:
         fm = FUSED_MODE[mode]
-        for p in parts:
-            s.apply_fused(p, fm)
+        if lows is not None:
+            s.apply_fused_chain(parts, lows, fm)
+        else:
+            for p in parts:
+                s.apply_fused(p, fm)
         if fm != "f64":
             s.renormalize()   # the split-precision products drift the norm by ~1e-5
 
 
-def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0, partitioner: str = "best") -> dict:
+def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0, partitioner: str = "best",
+                 layout: str = "fixed") -> dict:
     """Infidelity and max |d amp| of the chosen path against the FP64 oracle."""
     from oracle.oracle import Oracle
     if mode == "permute":
@@ -132,13 +145,13 @@ def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0, part
         return {"n": n, "infidelity": None, "max_abs_err": float(np.abs(got - want).max()),
                 "bit_exact": bool(np.array_equal(got, want))}
     gates = random_layered(n, depth, seed=seed)
-    parts = make_parts(gates, mode, k, partitioner)[0] if mode != "pergate" else []
+    parts, _, lows = make_parts(gates, mode, k, partitioner, layout) if mode != "pergate" else ([], None, None)
     want = np.zeros(2 ** n, complex)
     want[0] = 1
     o = Oracle()
     o.apply_gates(want, gates)
     with capi.State(n) as s:
-        run_path(s, mode, gates, parts)
+        run_path(s, mode, gates, parts, lows)
         got = s.read()
     infid = max(0.0, 1.0 - abs(np.vdot(want, got)) ** 2)
     return {"n": n, "depth": depth, "infidelity": infid, "max_abs_err": float(np.abs(got - want).max())}
@@ -190,6 +203,9 @@ def main() -> int:
     ap.add_argument("--partitioner", default="best", choices=["best", "greedy", "dfs", "closure"],
                     help="acyclic partitioner (best: the one with fewer parts)")
     ap.add_argument("--pergate-layers", type=int, default=1, help="layers of the per-gate timing sample")
+    ap.add_argument("--layout", default="chain", choices=["chain", "fixed"],
+                    help="wide BF16-split parts (k >= 8): moving low-slot layout (chain) or qubits 0-4 "
+                         "in every part (fixed)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="hsv_tune setting for the timed state (repeatable), e.g. part_tiles_per_wg=2")
     ap.add_argument("--no-cpu", action="store_true")
@@ -204,13 +220,13 @@ def main() -> int:
     k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 13, "pergate": 1, "permute": 3}[args.mode]
     n, depth = args.n or (32 if args.mode == "permute" else 33), args.depth
     gates = random_layered(n, depth, seed=0)
-    partitioner_used = None
+    partitioner_used, lows = None, None
     if args.mode == "permute":
         parts = exchange_permutation(n, k)
     elif args.mode == "pergate":
         parts = []
     else:
-        parts, partitioner_used = make_parts(gates, args.mode, k, args.partitioner)
+        parts, partitioner_used, lows = make_parts(gates, args.mode, k, args.partitioner, args.layout)
     S = 16
     total_upd = sum(gate_amplitudes(g, n) for g in gates)
     log(f"[bench] n={n} depth={depth} gates={len(gates)} mode={args.mode} k={k} parts={len(parts)}")
@@ -220,7 +236,7 @@ def main() -> int:
         for key, value in tune:
             s.tune(key, value)
         for w in range(args.warmup):
-            run_path(s, args.mode, gates, parts)
+            run_path(s, args.mode, gates, parts, lows)
             s.synchronize()
             log(f"[bench] warmup {w} done")
         s.set_timing(True)
@@ -228,7 +244,7 @@ def main() -> int:
         s.synchronize()
         t0 = time.perf_counter()
         for i in range(args.steps):
-            run_path(s, args.mode, gates, parts)
+            run_path(s, args.mode, gates, parts, lows)
             log(f"[bench] step {i} queued")
         s.synchronize()
         t1 = time.perf_counter()
@@ -238,8 +254,14 @@ def main() -> int:
         # per-launch durations: one per timed call (pass) in each step
         per_step = len(times) // max(1, args.steps)
         launches = times[: per_step * args.steps].reshape(args.steps, per_step)
+        restore_ms = None
+        if lows is not None and per_step == len(parts) + 1:
+            # the moving layout ends with one permutation pass back to the
+            # standard layout: reported on its own, not in the part average
+            restore_ms = float(launches[:, -1].mean())
+            launches = launches[:, :-1]
         kern_ms = float(launches.mean())
-        passes = launches.shape[1]
+        passes = per_step
         log(f"[bench] TTS {tts:.3f} s, {passes} passes/step, mean pass {kern_ms:.3f} ms")
 
         full_verify = None
@@ -247,7 +269,7 @@ def main() -> int:
             # the FP64 register part path on a second full-size state (two
             # 2^33 complex128 states fit in HBM together): infidelity of the
             # timed result at the benchmark's own size
-            ref_parts, _ = make_parts(gates, "part", 12, "best")
+            ref_parts, _, _ = make_parts(gates, "part", 12, "best")
             with capi.State(n) as ref:
                 run_path(ref, "part", gates, ref_parts)
                 ref.synchronize()
@@ -318,7 +340,8 @@ def main() -> int:
                 "per_launch": f"8 x 2^k x 2^{n} algorithmic flops (k={kk}) / {kern_ms:.3f} ms",
                 "hbm_gbs": 2 * S * 2 ** n / (kern_ms * 1e-3) / 1e9}
 
-    verify = None if args.no_verify else verify_small(args.mode, k, depth, partitioner=args.partitioner)
+    verify = None if args.no_verify else verify_small(args.mode, k, depth, partitioner=args.partitioner,
+                                                      layout=args.layout)
     if verify:
         log(f"[bench] verify vs FP64 oracle at n={verify['n']}: infidelity {verify['infidelity']}, "
             f"max|d| {verify['max_abs_err']:.2e}")
@@ -334,6 +357,8 @@ def main() -> int:
         "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, complex128 state "
                                 f"({2 ** n * 16 / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
                     "mode": args.mode, "k": k, "passes": passes, "partitioner": partitioner_used,
+                    **({"layout": "chain (moving low slots, one restoring permutation pass)",
+                        "layout_restore_ms": restore_ms} if lows is not None else {}),
                     **({"tune": dict(tune)} if tune else {})}
                    if args.mode != "permute" else
                    {"workload": f"exchange permutation of a {n}-qubit complex128 state ({2 ** n * 16 / 1e9:.1f} GB): "

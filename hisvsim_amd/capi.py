@@ -88,6 +88,7 @@ SIGNATURES = {
     "hsv_apply_gate": (ctypes.c_int, [_P, _G]),
     "hsv_apply_part": (ctypes.c_int, [_P, _G, ctypes.c_int, ctypes.c_int]),
     "hsv_apply_fused": (ctypes.c_int, [_P, _G, ctypes.c_int, ctypes.c_int]),
+    "hsv_apply_fused_chain": (ctypes.c_int, [_P, _G, ctypes.c_int, _I, ctypes.c_int, _I, ctypes.c_int]),
     "hsv_permute_qubits": (ctypes.c_int, [_P, _I]),
     "hsv_read": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
     "hsv_write": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
@@ -248,6 +249,22 @@ class State:
         m = FUSED_MODES[mode]
         arr = gate_array(gates)
         _check(lib().hsv_apply_fused(self._h, arr, len(gates), m))
+
+    def apply_fused_chain(self, parts: Sequence[Sequence], lows: Sequence[Sequence[int]],
+                          mode: str = "bf16split") -> None:
+        """Parts in sequence with a moving qubit layout (hsv_apply_fused_chain):
+        lows[p] are the 5 qubits in slots 0-4 while part p runs (lows[0] =
+        qubits 0-4); each must be a qubit of the previous part or its lows."""
+        if len(lows) != len(parts):
+            raise ValueError("one lows entry per part")
+        flat = [g for p in parts for g in p]
+        arr = gate_array(flat)
+        sizes = (ctypes.c_int * max(1, len(parts)))(*[len(p) for p in parts])
+        lw = [int(q) for low in lows for q in low]
+        if any(len(low) != 5 for low in lows):
+            raise ValueError("lows: 5 qubits per part")
+        larr = (ctypes.c_int * max(1, len(lw)))(*lw)
+        _check(lib().hsv_apply_fused_chain(self._h, arr, len(flat), sizes, len(parts), larr, FUSED_MODES[mode]))
 
     def permute_qubits(self, new_slot: Sequence[int]) -> None:
         arr = (ctypes.c_int * self.n)(*[int(x) for x in new_slot])

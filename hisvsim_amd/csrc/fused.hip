@@ -431,6 +431,11 @@ struct PartBfStep {
 };
 struct PartBfTables {
   uint64_t alo[64], ahi[128];   // address offset of tile index bits 0-5 / 6-12
+  // A layout change moves qubits between the tile's slots: the element
+  // stored at tile position u is the one loaded at position plo[u & 63] |
+  // phi[u >> 6] (the identity without a change), so the stores stay as
+  // coalesced as the loads.
+  uint16_t plo[64], phi[128];
   uint8_t swz[512];             // h(t >> 4): XOR into the low 5 bits of t
 };
 // 512 threads: 8 amplitudes per thread for 12-qubit tiles, 16 for 13.
@@ -466,6 +471,7 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
   __shared__ float2 ys[T];
   __shared__ PartBfStep st[kPartBfMaxSteps];
   __shared__ uint64_t alo[64], ahi[128];
+  __shared__ uint16_t plo[64], phi[128];
   __shared__ uint8_t swz[T >> 4];
   if constexpr (!STREAM_U)
     for (int i = threadIdx.x; i < nsteps * STEP_FRAGS; i += NTH) su[i] = ufrag[i];
@@ -476,6 +482,8 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
   }
   for (int i = threadIdx.x; i < 64; i += NTH) alo[i] = tabs->alo[i];
   for (int i = threadIdx.x; i < 128; i += NTH) ahi[i] = tabs->ahi[i];
+  for (int i = threadIdx.x; i < 64; i += NTH) plo[i] = tabs->plo[i];
+  for (int i = threadIdx.x; i < 128; i += NTH) phi[i] = tabs->phi[i];
   for (int i = threadIdx.x; i < (T >> 4); i += NTH) swz[i] = tabs->swz[i];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -574,14 +582,15 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
       }
     }
     __syncthreads();
-#pragma unroll
+#pragma unroll 8
     for (int j = 0; j < PER; ++j) {
-      const int t = threadIdx.x + NTH * j;
+      const int u = threadIdx.x + NTH * j;
+      const int t = plo[u & 63] | phi[u >> 6];
       const float2 y = ys[t ^ swz[t >> 4]];
       V o;
       o.x = (R)y.x;
       o.y = (R)y.y;
-      st_stream<R>(&a[base + alo[t & 63] + ahi[t >> 6]], o);
+      st_stream<R>(&a[base + alo[u & 63] + ahi[u >> 6]], o);
     }
   }
 }
@@ -841,8 +850,8 @@ void choose_part_swizzle(int KT, const std::vector<PartSwizzleKey>& keys, uint32
 }
 
 template <typename R, int KT>
-void launch_part_bf(void* amp, const unsigned char* dev, size_t steps_off, size_t frag_off, int nsteps, int grid,
-                    hipStream_t st, uint64_t ntiles, uint64_t outer, int tpw, int* rc) {
+void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t steps_off, size_t frag_off,
+                    int nsteps, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer, int tpw, int* rc) {
   const size_t lds = KT >= 13 ? 0 : size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);  // 13: fragments streamed
   auto kern = &k_part_bf<R, KT>;
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -851,18 +860,33 @@ void launch_part_bf(void* amp, const unsigned char* dev, size_t steps_off, size_
     return;
   }
   hipLaunchKernelGGL(kern, grid, part_bf_threads(KT), lds, st, static_cast<typename VecT<R>::type*>(amp),
-                     reinterpret_cast<const PartBfTables*>(dev),
+                     reinterpret_cast<const PartBfTables*>(dev + tabs_off),
                      reinterpret_cast<const PartBfStep*>(dev + steps_off),
                      reinterpret_cast<const bf8*>(dev + frag_off), nsteps, ntiles, outer, tpw);
 }
 
 // A part of 8..13 qubits on the BF16-split path (see k_part_bf).
+// tile_qubits: the (logical) qubits the tile must hold, at least those of
+// the gates. Logical qubit q sits in slot phys[q] (q when phys is null) and
+// the store leaves it in slot phys_after[q] (unchanged when null): a layout
+// change that must keep every qubit outside the tile where it is.
 int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
-                  const std::vector<Lowered>& gates, uint64_t pmask, int tpw, std::string* err) {
+                  const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
+                  const int* phys_after, int tpw, std::string* err) {
+  int ph[64], pa[64], inv[64];
+  for (int q = 0; q < n; ++q) ph[q] = phys ? phys[q] : q;
+  for (int q = 0; q < n; ++q) {
+    pa[q] = phys_after ? phys_after[q] : ph[q];
+    inv[ph[q]] = q;
+  }
+  uint64_t pmask = 0;
+  for (int q = 0; q < n; ++q)
+    if (tile_qubits >> q & 1) pmask |= 1ull << ph[q];
   const int K = __builtin_popcountll(pmask);
-  // The tile is the part's qubits padded with the lowest free ones. It should
-  // hold qubits 0-4, so that each wave's loads and stores cover 512-byte
-  // runs: a part that fits 12 qubits only without them takes a 13-qubit tile.
+  // The tile is the part's qubits padded with the lowest free slots. It
+  // should hold slots 0-4, so that each wave's loads and stores cover
+  // 512-byte runs: a part that fits 12 qubits only without them takes a
+  // 13-qubit tile.
   const uint64_t low5 = (1ull << std::min(n, 5)) - 1;
   const int KT = __builtin_popcountll(pmask | low5) <= 12 ? 12 : 13;
   if (K > 13) {
@@ -875,10 +899,28 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   }
   uint64_t tmask = pmask;
   for (int b = 0; b < n && __builtin_popcountll(tmask) < KT; ++b) tmask |= 1ull << b;
-  int slot[13];
+  int slot[13], sslot[13];
+  uint64_t smask = 0;
   for (int b = 0, j = 0; b < 64 && j < KT; ++b)
-    if (tmask >> b & 1) slot[j++] = b;
-  auto inner = [&](int q) { return __builtin_popcountll(tmask & ((1ull << q) - 1)); };
+    if (tmask >> b & 1) {
+      slot[j] = b;
+      sslot[j] = pa[inv[b]];
+      smask |= 1ull << sslot[j];
+      ++j;
+    }
+  for (int q = 0; q < n; ++q)
+    if (!(tmask >> ph[q] & 1) && pa[q] != ph[q]) smask = 0;
+  if (smask != tmask) {
+    *err = "the layout change moves a qubit into or out of the part's tile";
+    return HSV_ERR_ARG;
+  }
+  auto inner = [&](int q) { return __builtin_popcountll(tmask & ((1ull << ph[q]) - 1)); };
+  for (const Lowered& L : gates)
+    for (int i = 0; i < L.nt + L.nc; ++i)
+      if (!(tmask >> ph[i < L.nt ? L.t[i] : L.c[i - L.nt]] & 1)) {
+        *err = "a gate acts on a qubit outside the part's tile";
+        return HSV_ERR_ARG;
+      }
   std::vector<uint64_t> qm;
   std::vector<const Lowered*> gl;
   for (const Lowered& L : gates) {
@@ -889,9 +931,11 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     qm.push_back(m);
     gl.push_back(&L);
   }
-  if (qm.empty()) return HSV_OK;  // identities only
-  std::vector<std::vector<int>> groups = closure_groups(qm, 5);
-  if (groups.empty()) {
+  bool moves = false;
+  for (int j = 0; j < KT; ++j) moves = moves || sslot[j] != slot[j];
+  if (qm.empty() && !moves) return HSV_OK;  // identities only
+  std::vector<std::vector<int>> groups = qm.empty() ? std::vector<std::vector<int>>() : closure_groups(qm, 5);
+  if (groups.empty() && !qm.empty()) {
     *err = "a gate acts on more than 5 qubits: the wide bf16 fused path cannot take it";
     return HSV_ERR_LIMIT;
   }
@@ -907,7 +951,7 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     int rbit[5], cbit[8];
     std::vector<int> qubits;
     for (int b = 0, r = 0, c = 0; b < KT; ++b) {
-      if (gm >> b & 1) { rbit[r++] = b; qubits.push_back(slot[b]); }
+      if (gm >> b & 1) { rbit[r++] = b; qubits.push_back(inv[slot[b]]); }
       else cbit[c++] = b;
     }
     std::vector<cd> U;
@@ -961,25 +1005,44 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     for (auto& x : S.ctl) x = (uint16_t)(x ^ hmap(x));
     for (auto& x : S.cth) x = (uint16_t)(x ^ hmap(x));
   }
-  PartBfTables T;
-  std::memset(&T, 0, sizeof(T));
-  for (int i = 0; i < (1 << (KT - 4)); ++i) T.swz[i] = (uint8_t)hmap(uint32_t(i) << 4);
+  // two table sets: T0 stores where it loads (passes before the last, when
+  // the steps need several), T applies the layout change
+  PartBfTables T0, T;
+  std::memset(&T0, 0, sizeof(T0));
+  for (int i = 0; i < (1 << (KT - 4)); ++i) T0.swz[i] = (uint8_t)hmap(uint32_t(i) << 4);
   for (int v = 0; v < 128; ++v)
     for (int j = 0; j < 7; ++j) {
       if (!(v >> j & 1)) continue;
-      if (v < 64 && j < 6) T.alo[v] += 1ull << slot[j];
-      if (6 + j < KT) T.ahi[v] += 1ull << slot[6 + j];
+      if (v < 64 && j < 6) T0.alo[v] += 1ull << slot[j];
+      if (6 + j < KT) T0.ahi[v] += 1ull << slot[6 + j];
+    }
+  for (int v = 0; v < 64; ++v) T0.plo[v] = (uint16_t)v;
+  for (int v = 0; v < 128; ++v) T0.phi[v] = (uint16_t)(v << 6);
+  // tile bit j's qubit moves to tile bit sig[j] (the tile position of its new slot)
+  int sig_inv[13];
+  for (int j = 0; j < KT; ++j)
+    for (int k = 0; k < KT; ++k)
+      if (slot[k] == sslot[j]) sig_inv[k] = j;
+  T = T0;
+  std::memset(T.plo, 0, sizeof(T.plo));
+  std::memset(T.phi, 0, sizeof(T.phi));
+  for (int v = 0; v < 128; ++v)
+    for (int k = 0; k < 7; ++k) {
+      if (!(v >> k & 1)) continue;
+      if (v < 64 && k < 6) T.plo[v] |= (uint16_t)(1u << sig_inv[k]);
+      if (6 + k < KT) T.phi[v] |= (uint16_t)(1u << sig_inv[6 + k]);
     }
   // LDS: the tile (FP32) + step tables + tile tables statically, 8 KB per step
   const size_t static_lds =
-      (size_t(1) << KT) * 8 + sizeof(PartBfStep) * kPartBfMaxSteps + 192 * 8 + (size_t(1) << (KT - 4));
+      (size_t(1) << KT) * 8 + sizeof(PartBfStep) * kPartBfMaxSteps + 192 * 8 + 192 * 2 + (size_t(1) << (KT - 4));
   const int max_steps = KT >= 13 ? kPartBfMaxSteps
                                  : (int)std::min<size_t>(kPartBfMaxSteps, (160 * 1024 - static_lds) / (2 * 4 * 64 * 16));
-  const size_t steps_off = sizeof(PartBfTables);
+  const size_t steps_off = 2 * sizeof(PartBfTables);
   const size_t frag_off = (steps_off + sizeof(PartBfStep) * ns + 255) & ~size_t(255);
   const size_t bytes = frag_off + frags.size() * 2;
   std::vector<unsigned char> host(bytes);
-  std::memcpy(host.data(), &T, sizeof(T));
+  std::memcpy(host.data(), &T0, sizeof(T0));
+  std::memcpy(host.data() + sizeof(T0), &T, sizeof(T));
   std::memcpy(host.data() + steps_off, steps.data(), sizeof(PartBfStep) * ns);
   std::memcpy(host.data() + frag_off, frags.data(), frags.size() * 2);
   if (int rc = ensure_scratch(scratch, bytes, stream, err)) return rc;
@@ -995,15 +1058,17 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   const auto* dev = static_cast<const unsigned char*>(scratch->ptr);
   const bool c128 = dtype == HSV_C128;
   int rc = HSV_OK;
-  // more steps than fit in LDS at once: one pass per chunk of steps
-  for (int s0 = 0; s0 < ns && rc == HSV_OK; s0 += max_steps) {
+  // more steps than fit in LDS at once: one pass per chunk of steps, the
+  // last one with the layout change (a layout change alone: one pass, no steps)
+  for (int s0 = 0; (s0 < ns || (ns == 0 && s0 == 0)) && rc == HSV_OK; s0 += std::max(max_steps, 1)) {
     const int cnt = std::min(max_steps, ns - s0);
+    const size_t to = s0 + cnt >= ns ? sizeof(PartBfTables) : 0;
     const size_t so = steps_off + sizeof(PartBfStep) * s0;
     const size_t fo = frag_off + size_t(s0) * 2 * 4 * 64 * 16;
-    if (KT == 12) c128 ? launch_part_bf<double, 12>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
-                       : launch_part_bf<float, 12>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
-    else c128 ? launch_part_bf<double, 13>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
-              : launch_part_bf<float, 13>(amp, dev, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
+    if (KT == 12) c128 ? launch_part_bf<double, 12>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
+                       : launch_part_bf<float, 12>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
+    else c128 ? launch_part_bf<double, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
+              : launch_part_bf<float, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
   }
   if (rc) { *err = "wide bf16 fused kernel attribute setup failed"; return rc; }
   const hipError_t e = hipGetLastError();
@@ -1117,6 +1182,12 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
 
 }  // namespace
 
+int fused_part_bf_layout(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
+                         const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
+                         const int* phys_after, const BfTune& bf, std::string* err) {
+  return part_apply_bf(amp, n, dtype, stream, scratch, gates, tile_qubits, phys, phys_after, bf.part_tpw, err);
+}
+
 int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
                 const std::vector<Lowered>& gates, int mode, std::string* err, const BfTune& bf) {
   if (mode != HSV_FUSED_F64 && mode != HSV_FUSED_F32SPLIT && mode != HSV_FUSED_BF16SPLIT) {
@@ -1131,7 +1202,7 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
       for (int i = 0; i < L.nc; ++i) pmask |= 1ull << L.c[i];
     }
     if (__builtin_popcountll(pmask) > 7)
-      return part_apply_bf(amp, n, dtype, stream, scratch, gates, pmask, bf.part_tpw, err);
+      return part_apply_bf(amp, n, dtype, stream, scratch, gates, pmask, nullptr, nullptr, bf.part_tpw, err);
   }
   std::vector<int> Q;
   std::vector<cd> Um;

@@ -35,6 +35,15 @@ struct BfTune {
 int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
                 const std::vector<Lowered>& gates, int mode, std::string* err, const BfTune& bf = BfTune());
 
+// One part on the BF16-split wide-part kernel (a 12- or 13-qubit tile that
+// holds slots 0-4) with a qubit layout change: logical qubit q sits in slot
+// phys[q] and the pass leaves it in slot phys_after[q]; only qubits of the
+// tile may move. tile_qubits: the logical qubits the tile must hold (at
+// least those of the gates).
+int fused_part_bf_layout(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
+                         const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
+                         const int* phys_after, const BfTune& bf, std::string* err);
+
 // The fused unitary itself (for tests and the cost model): qubits of the
 // part (sorted, padded with the lowest free qubits to at least min_qubits)
 // and the row-major matrix.

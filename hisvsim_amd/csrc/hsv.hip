@@ -478,6 +478,85 @@ int lower_all(const hsv_gate* gates, int num_gates, std::vector<Lowered>* out, c
 
 }  // namespace
 
+// In-place qubit permutation (amplitude bit s moves to bit new_slot[s]) as
+// two involutions: on each cycle c_0 -> c_1 -> ... -> c_{m-1} of the slot
+// map, the reflections s1: c_k -> c_{-k} and s2: c_k -> c_{1-k} (indices
+// mod m) compose to s2(s1(c_k)) = c_{k+1}. Two swap passes, no second buffer.
+static int permute_in_place(hsv_state* s, const int* new_slot) {
+  const int n = s->n;
+  std::vector<int> s1(n), s2(n);
+  for (int q = 0; q < n; ++q) s1[q] = s2[q] = q;
+  std::vector<char> done(n, 0);
+  for (int q0 = 0; q0 < n; ++q0) {
+    if (done[q0]) continue;
+    std::vector<int> cyc;
+    for (int q = q0; !done[q]; q = new_slot[q]) {
+      done[q] = 1;
+      cyc.push_back(q);
+    }
+    const int m = (int)cyc.size();
+    for (int k = 0; k < m; ++k) {
+      s1[cyc[k]] = cyc[(m - k) % m];
+      s2[cyc[k]] = cyc[(1 - k + m) % m];
+    }
+  }
+  if (int rc = begin_timed(s)) return rc;
+  for (const std::vector<int>* sig : {&s1, &s2}) {
+    BitPairs bp{};
+    for (int q = 0; q < n; ++q)
+      if ((*sig)[q] > q) {
+        bp.lo[bp.npairs] = (uint8_t)q;
+        bp.hi[bp.npairs] = (uint8_t)(*sig)[q];
+        ++bp.npairs;
+      }
+    if (!bp.npairs) continue;
+    const int grid = grid_for(s->len, 256, uint64_t(s->num_cu) * 16);
+    if (s->dtype == HSV_C128)
+      hipLaunchKernelGGL((k_swap_bits<double>), grid, 256, 0, s->stream, amp_as<double>(s), s->len, bp);
+    else
+      hipLaunchKernelGGL((k_swap_bits<float>), grid, 256, 0, s->stream, amp_as<float>(s), s->len, bp);
+    HIPCHK(hipGetLastError());
+  }
+  return end_timed(s);
+}
+
+// Out-of-place qubit permutation: amplitude bit s moves to bit new_slot[s].
+static int permute_impl(hsv_state* s, const int* new_slot) {
+  uint64_t seen = 0;
+  bool identity = true;
+  for (int q = 0; q < s->n; ++q) {
+    if (new_slot[q] < 0 || new_slot[q] >= s->n || (seen >> new_slot[q] & 1))
+      return fail(HSV_ERR_ARG, "new_slot is not a permutation of 0..n-1");
+    seen |= 1ull << new_slot[q];
+    identity = identity && new_slot[q] == q;
+  }
+  if (identity) return HSV_OK;
+  void* out = nullptr;
+  if (hipMalloc(&out, s->len * s->elem()) != hipSuccess)
+    return fail(HSV_ERR_ALLOC, "no room for the permutation's second buffer");
+  int* dslot = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&dslot), sizeof(int) * 64) != hipSuccess) {
+    hipFree(out);
+    return fail(HSV_ERR_ALLOC, "permutation table allocation failed");
+  }
+  HIPCHK(hipMemcpyAsync(dslot, new_slot, sizeof(int) * s->n, hipMemcpyHostToDevice, s->stream));
+  if (int rc = begin_timed(s)) return rc;
+  const int grid = grid_for(s->len, 256, uint64_t(s->num_cu) * 8);
+  if (s->dtype == HSV_C128)
+    hipLaunchKernelGGL((k_permute<double>), grid, 256, 0, s->stream, amp_as<double>(s),
+                       static_cast<double2*>(out), s->len, s->n, dslot);
+  else
+    hipLaunchKernelGGL((k_permute<float>), grid, 256, 0, s->stream, amp_as<float>(s),
+                       static_cast<float2*>(out), s->len, s->n, dslot);
+  HIPCHK(hipGetLastError());
+  if (int rc = end_timed(s)) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipFree(s->amp));
+  HIPCHK(hipFree(dslot));
+  s->amp = out;
+  return HSV_OK;
+}
+
 // ============================================================================
 extern "C" {
 
@@ -602,42 +681,138 @@ int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode
   return end_timed(s);
 }
 
+int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, const int* part_sizes,
+                          int num_parts, const int* lows, int mode) {
+  if (!s || num_gates < 0 || num_parts < 0 || (num_gates > 0 && !gates) ||
+      (num_parts > 0 && (!part_sizes || !lows)))
+    return fail(HSV_ERR_ARG, "bad arguments");
+  if (mode != HSV_FUSED_BF16SPLIT) return fail(HSV_ERR_ARG, "the part chain runs on the BF16-split path only");
+  const int n = s->n;
+  if (n < 12) return fail(HSV_ERR_LIMIT, "the part chain needs a state of at least 12 qubits");
+  long total = 0;
+  for (int p = 0; p < num_parts; ++p) {
+    if (part_sizes[p] < 0) return fail(HSV_ERR_ARG, "negative part size");
+    total += part_sizes[p];
+    uint64_t seen = 0;
+    for (int i = 0; i < 5; ++i) {
+      const int q = lows[5 * p + i];
+      if (q < 0 || q >= n || (seen >> q & 1)) return fail(HSV_ERR_ARG, "lows: 5 distinct qubits per part");
+      seen |= 1ull << q;
+    }
+  }
+  if (total != num_gates) return fail(HSV_ERR_ARG, "part sizes do not add up to num_gates");
+  std::vector<Lowered> Ls;
+  if (int rc = lower_all(gates, num_gates, &Ls, s)) return rc;
+  // the layouts before and after each part, worked out and checked first
+  std::vector<std::vector<int>> layout(num_parts + 1, std::vector<int>(n));
+  std::vector<uint64_t> tile(num_parts, 0);
+  for (int q = 0; q < n; ++q) layout[0][q] = q;
+  size_t off = 0;
+  for (int p = 0; p < num_parts; ++p) {
+    const std::vector<int>& phys = layout[p];
+    uint64_t& tq = tile[p];
+    for (int i = 0; i < 5; ++i) {
+      const int q = lows[5 * p + i];
+      if (phys[q] >= 5) return fail(HSV_ERR_ARG, "lows of part " + std::to_string(p) + " are not the qubits in slots 0-4");
+      tq |= 1ull << q;
+    }
+    for (size_t g = off; g < off + part_sizes[p]; ++g) {
+      const Lowered& L = Ls[g];
+      if (L.kind == K_IDENT) continue;
+      for (int i = 0; i < L.nt; ++i) tq |= 1ull << L.t[i];
+      for (int i = 0; i < L.nc; ++i) tq |= 1ull << L.c[i];
+    }
+    off += part_sizes[p];
+    std::vector<int>& after = layout[p + 1];
+    after = phys;
+    if (p + 1 < num_parts) {
+      // bring the next part's lows into slots 0-4, each swapping slots with
+      // a current occupant the next part does not list
+      const int* nl = lows + 5 * (p + 1);
+      uint64_t want = 0;
+      for (int i = 0; i < 5; ++i) {
+        if (!(tq >> nl[i] & 1))
+          return fail(HSV_ERR_ARG, "lows of part " + std::to_string(p + 1) + " must be qubits of part " +
+                                       std::to_string(p) + " or of its lows");
+        want |= 1ull << nl[i];
+      }
+      std::vector<int> spare;
+      for (int i = 0; i < 5; ++i)
+        if (!(want >> lows[5 * p + i] & 1)) spare.push_back(lows[5 * p + i]);
+      for (int i = 0; i < 5; ++i) {
+        const int q = nl[i];
+        if (after[q] < 5) continue;
+        const int o = spare.back();
+        spare.pop_back();
+        std::swap(after[q], after[o]);
+      }
+    } else {
+      // the last part sends each of its qubits whose own slot is among its
+      // slots home; the others fill the remaining slots in order
+      std::vector<int> slots, rest, free_slots;
+      for (int q = 0; q < n; ++q)
+        if (tq >> q & 1) slots.push_back(phys[q]);
+      std::sort(slots.begin(), slots.end());
+      uint64_t home = 0;  // slots whose own qubit is in the part
+      for (int sl : slots)
+        if (tq >> sl & 1) home |= 1ull << sl;
+      for (int sl : slots)
+        if (!(home >> sl & 1)) free_slots.push_back(sl);
+      for (int sl : slots) {
+        int q = 0;
+        while (phys[q] != sl) ++q;
+        if (home >> q & 1) after[q] = q;
+        else rest.push_back(q);
+      }
+      for (size_t i = 0; i < rest.size(); ++i) after[rest[i]] = free_slots[i];
+    }
+  }
+  // back to the standard layout: the bit in slot phys[q] moves to slot q;
+  // one tile pass without gates when the qubits still away from home and
+  // those in slots 0-4 fit a tile, else two in-place swap passes
+  auto restore = [&](const std::vector<int>& phys) -> int {
+    std::vector<int> new_slot(n), ident(n);
+    uint64_t away = 0, lowq = 0;
+    for (int q = 0; q < n; ++q) {
+      new_slot[phys[q]] = q;
+      ident[q] = q;
+      if (phys[q] != q) away |= 1ull << q;
+      if (phys[q] < 5) lowq |= 1ull << q;
+    }
+    if (!away) return HSV_OK;
+    if (__builtin_popcountll(away | lowq) <= 13) {
+      if (int rc = begin_timed(s)) return rc;
+      std::string err;
+      if (int rc = fused_part_bf_layout(s->amp, n, s->dtype, s->stream, &s->fused_scratch, {}, away | lowq,
+                                        phys.data(), ident.data(), s->bf, &err))
+        return fail(rc, "layout restore: " + err);
+      return end_timed(s);
+    }
+    return permute_in_place(s, new_slot.data());
+  };
+  off = 0;
+  for (int p = 0; p < num_parts; ++p) {
+    const std::vector<Lowered> part(Ls.begin() + off, Ls.begin() + off + part_sizes[p]);
+    off += part_sizes[p];
+    if (int rc = begin_timed(s)) return rc;
+    std::string err;
+    if (int rc = fused_part_bf_layout(s->amp, n, s->dtype, s->stream, &s->fused_scratch, part, tile[p],
+                                      layout[p].data(), layout[p + 1].data(), s->bf, &err)) {
+      restore(layout[p]);
+      return fail(rc, "part " + std::to_string(p) + ": " + err);
+    }
+    if (int rc = end_timed(s)) return rc;
+    g_part_count++;
+    g_gate_count += part_sizes[p];
+  }
+  return restore(layout[num_parts]);
+}
+
 int hsv_permute_qubits(hsv_state* s, const int* new_slot) {
   if (!s || !new_slot) return fail(HSV_ERR_ARG, "null argument");
-  uint64_t seen = 0;
-  bool identity = true;
-  for (int q = 0; q < s->n; ++q) {
-    if (new_slot[q] < 0 || new_slot[q] >= s->n || (seen >> new_slot[q] & 1))
-      return fail(HSV_ERR_ARG, "new_slot is not a permutation of 0..n-1");
-    seen |= 1ull << new_slot[q];
-    identity = identity && new_slot[q] == q;
-  }
-  if (identity) return HSV_OK;
-  void* out = nullptr;
-  if (hipMalloc(&out, s->len * s->elem()) != hipSuccess)
-    return fail(HSV_ERR_ALLOC, "no room for the permutation's second buffer");
-  int* dslot = nullptr;
-  if (hipMalloc(reinterpret_cast<void**>(&dslot), sizeof(int) * 64) != hipSuccess) {
-    hipFree(out);
-    return fail(HSV_ERR_ALLOC, "permutation table allocation failed");
-  }
-  HIPCHK(hipMemcpyAsync(dslot, new_slot, sizeof(int) * s->n, hipMemcpyHostToDevice, s->stream));
-  if (int rc = begin_timed(s)) return rc;
-  const int grid = grid_for(s->len, 256, uint64_t(s->num_cu) * 8);
-  if (s->dtype == HSV_C128)
-    hipLaunchKernelGGL((k_permute<double>), grid, 256, 0, s->stream, amp_as<double>(s),
-                       static_cast<double2*>(out), s->len, s->n, dslot);
-  else
-    hipLaunchKernelGGL((k_permute<float>), grid, 256, 0, s->stream, amp_as<float>(s),
-                       static_cast<float2*>(out), s->len, s->n, dslot);
-  HIPCHK(hipGetLastError());
-  if (int rc = end_timed(s)) return rc;
-  HIPCHK(hipStreamSynchronize(s->stream));
-  HIPCHK(hipFree(s->amp));
-  HIPCHK(hipFree(dslot));
-  s->amp = out;
-  return HSV_OK;
+  return permute_impl(s, new_slot);
 }
+
 
 int hsv_read(hsv_state* s, uint64_t offset, uint64_t count, double* host_out) {
   if (!s || (count && !host_out) || offset > s->len || count > s->len - offset)

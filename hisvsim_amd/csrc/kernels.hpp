@@ -427,6 +427,35 @@ __global__ __launch_bounds__(256) void k_zero_state(typename VecT<R>::type* __re
 // out[sigma(i)] = in[i]; sigma moves bit s to bit new_slot[s]. sigma(i) is
 // looked up per 9-bit chunk of i (4 chunks: n <= 36) from tables built per
 // workgroup in LDS (16 KB).
+// ---- in-place qubit involution ---------------------------------------------
+// The qubits of each pair (lo[p], hi[p]) trade slots: amplitudes a[i] and
+// a[sigma(i)] swap, sigma exchanging the bit values of every pair. Each
+// moved amplitude is read and written once; a pair is handled by the thread
+// of its lower index. Any qubit permutation is two such involutions.
+struct BitPairs {
+  int npairs;
+  uint8_t lo[32], hi[32];
+};
+template <typename R>
+__global__ __launch_bounds__(256) void k_swap_bits(typename VecT<R>::type* __restrict__ a, uint64_t len,
+                                                   BitPairs bp) {
+  using V = typename VecT<R>::type;
+  const uint64_t stride = uint64_t(gridDim.x) * 256;
+  for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < len; i += stride) {
+    uint64_t j = i;
+    for (int p = 0; p < bp.npairs; ++p) {
+      const uint64_t d = ((i >> bp.lo[p]) ^ (i >> bp.hi[p])) & 1;
+      j ^= (d << bp.lo[p]) | (d << bp.hi[p]);
+    }
+    if (j > i) {
+      const V x = ldg<true>(&a[i]);
+      const V y = ldg<true>(&a[j]);
+      stg<true>(&a[i], y);
+      stg<true>(&a[j], x);
+    }
+  }
+}
+
 constexpr int kPermChunk = 9;
 template <typename R>
 __global__ __launch_bounds__(256) void k_permute(const typename VecT<R>::type* __restrict__ in,

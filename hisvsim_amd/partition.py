@@ -28,7 +28,7 @@ from __future__ import annotations
 
 import heapq
 from collections import OrderedDict
-from typing import Dict, List, Sequence
+from typing import Dict, List, Sequence, Tuple
 
 from .circuit import Gate
 
@@ -279,29 +279,7 @@ def closure_parts(gates: Sequence[Gate], k: int, always: Sequence[int] = (),
     remaining = list(range(len(gates)))
     parts: List[List[Gate]] = []
     while remaining:
-        Q = set(base)
-        while len(Q) < k:
-            blocked: set = set()
-            cands: List[frozenset] = []
-            for i in remaining:
-                gq = qsets[i]
-                if gq <= Q:
-                    continue
-                add = gq - Q
-                if not (gq & blocked) and len(Q) + len(add) <= k and add not in cands:
-                    cands.append(add)
-                blocked |= gq
-                if len(cands) >= candidates:
-                    break
-            if not cands:
-                break
-            best, best_score = None, -1.0
-            for add in cands:
-                score = len(_absorb(qsets, remaining, Q | add)) / len(add)
-                if score > best_score:
-                    best, best_score = add, score
-            Q |= best
-        taken = _absorb(qsets, remaining, Q)
+        taken = _absorb(qsets, remaining, _grow(qsets, remaining, base, k, candidates))
         if not taken:
             raise ValueError(f"gate {gates[remaining[0]]} does not fit a part of {k} qubits "
                              f"that already holds {sorted(base)}")
@@ -309,6 +287,82 @@ def closure_parts(gates: Sequence[Gate], k: int, always: Sequence[int] = (),
         parts.append([gates[i] for i in taken])
         remaining = [i for i in remaining if i not in done]
     return parts
+
+
+def _grow(qsets: Sequence[frozenset], remaining: Sequence[int], base: set, k: int,
+          candidates: int = 24) -> set:
+    """The qubit-set growth of ``closure_parts`` for one part, from ``base``."""
+    Q = set(base)
+    while len(Q) < k:
+        blocked: set = set()
+        cands: List[frozenset] = []
+        for i in remaining:
+            gq = qsets[i]
+            if gq <= Q:
+                continue
+            add = gq - Q
+            if not (gq & blocked) and len(Q) + len(add) <= k and add not in cands:
+                cands.append(add)
+            blocked |= gq
+            if len(cands) >= candidates:
+                break
+        if not cands:
+            break
+        best, best_score = None, -1.0
+        for add in cands:
+            score = len(_absorb(qsets, remaining, Q | add)) / len(add)
+            if score > best_score:
+                best, best_score = add, score
+        Q |= best
+    return Q
+
+
+def chain_parts(gates: Sequence[Gate], k: int, low: int = 5, lookahead: int = 7,
+                candidates: int = 24) -> Tuple[List[List[Gate]], List[List[int]]]:
+    """Parts for a moving qubit layout (hsv_apply_fused_chain): part p is
+    grown by the ``closure_parts`` method from the ``low`` qubits that sit in
+    the lowest slots while it runs (lows[p]; lows[0] = qubits 0..low-1), so
+    its tile - its qubits and lows[p] - has at most k qubits. lows[p + 1] is
+    chosen among part p's tile: of the ``lookahead`` qubits whose next gates
+    come first, the ``low`` that let the next part absorb the most gates.
+    Unlike fixed low qubits, the low slots then hold qubits the next part
+    uses. Returns (parts, lows); the parts are acyclic as for closure_parts."""
+    from itertools import combinations
+    if low > k:
+        raise ValueError("more low qubits than the part width")
+    qsets = [frozenset(gate_qubits(g)) for g in gates]
+    nq = max((max(q) for q in qsets if q), default=-1) + 1
+    if nq < low:
+        raise ValueError(f"the circuit has {nq} qubits: fewer than the {low} low slots")
+    remaining = list(range(len(gates)))
+    lowset = set(range(low))
+    parts: List[List[Gate]] = []
+    lows: List[List[int]] = []
+    while remaining:
+        taken = _absorb(qsets, remaining, _grow(qsets, remaining, lowset, k, candidates))
+        if not taken:
+            raise ValueError(f"gate {gates[remaining[0]]} does not fit a part of {k} qubits "
+                             f"that already holds {sorted(lowset)}")
+        done = set(taken)
+        parts.append([gates[i] for i in taken])
+        lows.append(sorted(lowset))
+        tile = set(lowset).union(*[qsets[i] for i in taken])
+        remaining = [i for i in remaining if i not in done]
+        if not remaining:
+            break
+        first: dict = {}
+        for pos, i in enumerate(remaining):
+            for q in qsets[i]:
+                if q in tile and q not in first:
+                    first[q] = pos
+        order = sorted(tile, key=lambda q: (first.get(q, len(remaining)), q))
+        best = None
+        for cand in combinations(order[:max(lookahead, low)], low):
+            got = len(_absorb(qsets, remaining, _grow(qsets, remaining, set(cand), k, candidates)))
+            if best is None or got > best[0]:
+                best = (got, set(cand))
+        lowset = best[1]
+    return parts, lows
 
 
 def merge_parts(parts: Sequence[Sequence[Gate]], k: int) -> List[List[Gate]]:

@@ -96,6 +96,24 @@ int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_s
  * to at least 5 qubits and needs at least 4 state qubits outside the part. */
 int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode);
 
+/* Not in the reference: a sequence of parts on the BF16-split path
+ * (mode HSV_FUSED_BF16SPLIT only) with a moving qubit layout, in the role
+ * of the reference's part loop (qasm_assembler_standalone.py:1783-2014, one
+ * gather_and_execute_on per part). gates holds the parts one after another,
+ * part p being part_sizes[p] gates; lows[5p .. 5p+4] names the 5 qubits that
+ * sit in slots 0-4 while part p runs (for p = 0: qubits 0-4, in any order).
+ * Each part is one pass of a 12- or 13-qubit tile holding slots 0-4 (so its
+ * loads and stores cover 512-byte runs) and the part's qubits; the pass also
+ * moves the next part's lows, which must be qubits of part p or of its lows,
+ * into slots 0-4. So a part spends no tile slots on qubits it does not use
+ * beyond those 5, and parts can be wider in useful qubits than with fixed
+ * low qubits. Two in-place swap passes (the layout change as two
+ * involutions) restore the standard layout at the end. Argument errors are
+ * found before any pass; a part that fails later (a gate on more than 5
+ * qubits) leaves the parts before it applied, in the standard layout. */
+int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, const int* part_sizes,
+                          int num_parts, const int* lows, int mode);
+
 /* Replaces the qubit exchange of SvSimMPI::gather_qubits / gather_slots
  * (svsim-mpi.hpp:123-173, mpi_redistributer.hpp:265-344) on one device:
  * amplitude bit s moves to bit new_slot[s]. */

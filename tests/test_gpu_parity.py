@@ -191,6 +191,47 @@ def test_fused_bf16split_wide_part_off_the_low_qubits(orc, dtype):
     assert np.abs(got - want).max() <= 1e-6
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("n,depth,k", [(14, 10, 12), (16, 12, 13), (20, 14, 13), (18, 8, 12)])
+def test_fused_chain_matches_oracle(orc, n, depth, k, dtype):
+    """Parts with a moving qubit layout (hsv_apply_fused_chain): each pass
+    also brings the next part's lows into slots 0-4, and one permutation
+    restores the standard layout; within 1e-6 of the FP64 oracle."""
+    from hisvsim_amd.partition import chain_parts
+    gates = random_layered(n, depth, seed=n + depth)
+    parts, lows = chain_parts(gates, k)
+    assert any(sorted(low) != [0, 1, 2, 3, 4] for low in lows)   # the layout does move
+    a = rand_state(n, depth)
+    want = a.copy()
+    orc.apply_gates(want, gates)
+    got = gpu_run(n, a, lambda s: [s.apply_fused_chain(parts, lows), s.renormalize()], dtype)
+    assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
+    assert np.abs(got - want).max() <= 1e-6
+
+
+def test_fused_chain_rejects_bad_lows(orc):
+    """Argument errors are found before any pass: the state is unchanged."""
+    from hisvsim_amd.partition import chain_parts
+    n = 14
+    gates = random_layered(n, 6, seed=3)
+    parts, lows = chain_parts(gates, 12)
+    a = rand_state(n, 1)
+    with capi.State(n) as s:
+        s.write(a)
+        bad_first = [[1, 2, 3, 4, 5]] + lows[1:]
+        with pytest.raises(capi.HsvError, match="ARG"):
+            s.apply_fused_chain(parts, bad_first)
+        tile0 = set(lows[0]).union(*[set(g.qubits()) for g in parts[0]])
+        outside = [q for q in range(n) if q not in tile0]
+        if len(parts) > 1 and outside:
+            bad_next = [lows[0], [outside[0]] + lows[1][1:]] + lows[2:]
+            with pytest.raises(capi.HsvError, match="ARG"):
+                s.apply_fused_chain(parts, bad_next)
+        with pytest.raises(capi.HsvError, match="ARG"):
+            s.apply_fused_chain(parts, lows, "f64")
+        assert np.array_equal(s.read(), a)
+
+
 def test_fused_bf16split_wide_rejects_six_qubit_gate():
     g = Gate("SWAP", (0, 1), (2, 3, 4, 5))   # 6 qubits: wider than a 5-qubit step
     with capi.State(14) as s:

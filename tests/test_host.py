@@ -21,7 +21,8 @@ from conftest import ROOT, rand_state
 from gatesets import ALL_ONE, ALL_TWO, params_for, random_circuit
 from hisvsim_amd import capi
 from hisvsim_amd.circuit import Gate, random_layered
-from hisvsim_amd.partition import (check_parts, dfs_order, dfs_parts, gate_dag, greedy_parts, layered_parts,
+from hisvsim_amd.partition import (chain_parts, check_parts, closure_parts, dfs_order, dfs_parts, gate_dag,
+                                   greedy_parts, layered_parts,
                                    merge_parts, nat_parts)
 from test_oracle_golden import full_operator, matrix_of
 
@@ -187,6 +188,36 @@ def test_merge_parts_never_adds_parts():
 def test_greedy_beats_natural_order_on_layered():
     gates = random_layered(24, 20, seed=1)
     assert len(greedy_parts(gates, 8)) < len(nat_parts(gates, 8)) / 1.5
+
+
+@pytest.mark.parametrize("n,depth,k", [(16, 12, 12), (20, 10, 13), (24, 6, 13)])
+def test_chain_parts_valid(n, depth, k):
+    """Moving-layout parts: an acyclic k-bounded partition whose tiles (part
+    qubits + lows) fit k, lows[0] = qubits 0-4, each later lows drawn from
+    the previous tile; no more parts than with fixed low qubits."""
+    gates = random_layered(n, depth, seed=n)
+    parts, lows = chain_parts(gates, k)
+    check_parts(gates, parts, k)
+    assert sorted(lows[0]) == [0, 1, 2, 3, 4]
+    for p, (part, low) in enumerate(zip(parts, lows)):
+        assert len(set(low)) == 5
+        tile = set(low).union(*[set(g.qubits()) for g in part])
+        assert len(tile) <= k
+        if p + 1 < len(parts):
+            assert set(lows[p + 1]) <= tile
+    assert len(parts) <= len(closure_parts(gates, k, always=range(5)))
+
+
+def test_chain_parts_apply_like_the_circuit(orc):
+    """Applying the chain's parts in order equals the circuit (oracle, n=12)."""
+    gates = random_layered(12, 8, seed=4)
+    parts, _ = chain_parts(gates, 9)
+    a = rand_state(12, 2)
+    want, got = a.copy(), a.copy()
+    orc.apply_gates(want, gates)
+    for p in parts:
+        orc.apply_gates(got, p)
+    assert np.abs(got - want).max() <= 1e-12
 
 
 def test_check_parts_detects_reordering():
