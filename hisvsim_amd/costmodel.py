@@ -6,14 +6,21 @@ and the partition; the plain partitioner (natural order, the reference's
 model.
 
 Features of a part, for a state of n qubits and a kernel mode (register
-part kernel, fused FP64 unitary or fused BF16-split unitary):
+part kernel, fused FP64 unitary, fused BF16-split unitary - one block of
+5..7 qubits, or a 12/13-qubit tile of 5-qubit steps for wider parts - or a
+part of the moving-layout chain, whose tile adds the part's low-slot
+qubits):
   part qubits k, the width the device runs it at (k plus the low qubits it
-  adds, or the fused path's padded width), gate count, 1-qubit and
-  multi-qubit gate counts, the multi-qubit fraction, min / max / spread of
-  the qubit indices, whether it touches the top 3 (high-order) qubits, two
-  mode flags and n.
+  adds, the fused path's padded width, or the tile width), gate count,
+  1-qubit and multi-qubit gate counts, the multi-qubit fraction, min / max /
+  spread of the qubit indices, whether it touches the top 3 (high-order)
+  qubits, two mode flags, n, and the number of fused blocks the pass
+  applies (0 for the register kernel, 1 for a 5..7-qubit unitary, the
+  5-qubit steps of a wide part).
 Target: log(ms per pass scaled to a 2^30-amplitude state), so one model
-covers every n (a pass is memory-bound work proportional to 2^n).
+covers every n (a pass is memory-bound work proportional to 2^n). A chain
+plan adds its layout restore, priced per 2^30 amplitudes by the median of
+the restores timed during collection.
 """
 from __future__ import annotations
 
@@ -28,28 +35,36 @@ import numpy as np
 
 from . import capi
 from .circuit import Gate
-from .partition import closure_parts, dfs_parts, greedy_parts, nat_parts
+from .partition import chain_parts, closure_parts, dfs_parts, greedy_parts, nat_parts
 
-MODES = ("part", "fused64", "fusedbf16")
+MODES = ("part", "fused64", "fusedbf16", "chain")
 FUSED_PRECISION = {"fused64": "f64", "fusedbf16": "bf16split"}
 LOW_SLOTS = 5
-N_FEATURES = 13
+N_FEATURES = 14
+RESTORE_MS_PER_2_30 = 26.0   # used when a parameter file has no measured restore cost
 PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "costmodel_params.json")
 
 
-def part_features(part: Sequence[Gate], n: int, mode: str) -> np.ndarray:
-    qs = sorted({q for g in part for q in g.qubits()})
+def part_features(part: Sequence[Gate], n: int, mode: str, low: Sequence[int] = ()) -> np.ndarray:
+    """``low``: the chain part's low-slot qubits (mode "chain" only)."""
+    gq = {q for g in part for q in g.qubits()}
+    qs = sorted(gq | set(low))
     k = len(qs)
+    low5 = set(range(min(LOW_SLOTS, n)))
     if mode == "part":
-        width = len(set(qs) | set(range(min(LOW_SLOTS, n))))
+        width, blocks = len(set(qs) | low5), 0
+    elif mode == "chain" or (mode == "fusedbf16" and k > 7):
+        # the wide-part kernel's tile: 12 or 13 qubits holding slots 0-4
+        width = 12 if len(set(qs) | (low5 if mode != "chain" else set())) <= 12 else 13
+        blocks = len(closure_parts(part, 5)) if part else 0
     else:
-        width = max(4 if mode == "fused64" else 5, k)
+        width, blocks = max(4 if mode == "fused64" else 5, k), 1
     n1 = sum(1 for g in part if len(g.qubits()) == 1)
     n2 = len(part) - n1
     return np.array([
         k / 10.0, width / 10.0, len(part) / 20.0, n1 / 20.0, n2 / 20.0, n2 / max(1, len(part)),
         qs[0] / 30.0, qs[-1] / 30.0, (qs[-1] - qs[0]) / 30.0, float(qs[-1] >= n - 3),
-        float(mode == "fused64"), float(mode == "fusedbf16"), (n - 25) / 5.0,
+        float(mode == "fused64"), float(mode in ("fusedbf16", "chain")), (n - 25) / 5.0, blocks / 4.0,
     ], dtype=np.float32)
 
 
@@ -134,18 +149,22 @@ class MLP:
             d = json.load(f)
         m = cls(d["in"], d["hidden"])
         m.set_params(np.array(d["params"], np.float32))
+        m.restore_ms_per_2_30 = float(d.get("meta", {}).get("restore_ms_per_2_30", RESTORE_MS_PER_2_30))
         return m
 
 
 # ---- planning ------------------------------------------------------------------
 def candidate_specs() -> List[Tuple[str, int, str]]:
     """(mode, k, partitioner) candidates: the fused unitary at k = 5..7 (FP64
-    or BF16-split MFMA) and the part kernel at k = 8..13 (its parts already
-    hold the 5 low qubits), with the DAG-aware greedy, depth-first,
-    qubit-set closure or natural-order partitioner (k = 5..10 for natural
-    order on the part kernel). k is the block width the model picks."""
+    or BF16-split MFMA), wide BF16-split parts at k = 8..13 and the part
+    kernel at k = 8..13 (those parts already hold the 5 low qubits), with the
+    DAG-aware greedy, depth-first, qubit-set closure or natural-order
+    partitioner (k = 5..10 for natural order on the part kernel), and the
+    moving-layout chain at k = 12, 13. k is the block width the model picks."""
     specs = [("fused64", k, name) for k in range(5, 8) for name in ("greedy", "nat")]
     specs += [("fusedbf16", k, name) for k in range(5, 8) for name in ("closure", "dfs")]
+    specs += [("fusedbf16", k, "closure") for k in range(8, 14)]
+    specs += [("chain", k, "chain") for k in (12, 13)]
     specs += [("part", k, name) for k in range(8, 14) for name in ("greedy", "dfs", "closure")]
     specs += [("part", k, "nat") for k in range(5, 11)]
     return specs
@@ -155,9 +174,12 @@ def build_plan(gates: Sequence[Gate], n: int, spec: Tuple[str, int, str]) -> dic
     mode, k, name = spec
     low = range(min(LOW_SLOTS, n))
     fn = {"dfs": dfs_parts, "closure": closure_parts, "greedy": greedy_parts}
+    if mode == "chain":
+        parts, lows = chain_parts(gates, k, LOW_SLOTS)
+        return {"mode": mode, "k": k, "partitioner": name, "parts": parts, "lows": lows}
     if name == "nat":
         parts = nat_parts(gates, k)
-    elif mode == "part":
+    elif mode == "part" or (mode == "fusedbf16" and k >= 8):
         parts = fn[name](gates, max(k, len(low)), always=low)
     else:
         parts = fn[name](gates, k)
@@ -173,9 +195,16 @@ def fallback_plan(gates: Sequence[Gate], n: int, k: int = 10) -> dict:
     return {"mode": "part", "k": k, "partitioner": "nat", "parts": nat_parts(gates, k)}
 
 
+def plan_features(plan: dict, n: int) -> np.ndarray:
+    lows = plan.get("lows") or [()] * len(plan["parts"])
+    return np.stack([part_features(p, n, plan["mode"], low) for p, low in zip(plan["parts"], lows)])
+
+
 def predict_plan_ms(model: MLP, plan: dict, n: int) -> float:
-    X = np.stack([part_features(p, n, plan["mode"]) for p in plan["parts"]])
-    return float(sum(ms_of(float(t), n) for t in model.predict(X)))
+    ms = float(sum(ms_of(float(t), n) for t in model.predict(plan_features(plan, n))))
+    if plan["mode"] == "chain":
+        ms += getattr(model, "restore_ms_per_2_30", RESTORE_MS_PER_2_30) * 2.0 ** (n - 30)
+    return ms
 
 
 def choose_plan(model: MLP, gates: Sequence[Gate], n: int) -> Tuple[dict, List[dict]]:
@@ -187,16 +216,19 @@ def choose_plan(model: MLP, gates: Sequence[Gate], n: int) -> Tuple[dict, List[d
 
 
 def run_plan(s: capi.State, plan: dict, timed: bool = True) -> np.ndarray:
-    """Runs a plan on the state; returns each part's kernel time (ms)."""
+    """Runs a plan on the state; returns each part's kernel time (ms), and
+    for a chain plan its layout restore as one more entry."""
     if timed:
         s.set_timing(True)
         s.kernel_times()
-    for p in plan["parts"]:
+    if plan["mode"] == "chain":
+        s.apply_fused_chain(plan["parts"], plan["lows"], "bf16split")
+    for p in plan["parts"] if plan["mode"] != "chain" else ():
         if plan["mode"] in FUSED_PRECISION:
             s.apply_fused(p, FUSED_PRECISION[plan["mode"]])
         else:
             s.apply_part(p, 0)
-    if plan["mode"] == "fusedbf16":
+    if plan["mode"] in ("fusedbf16", "chain"):
         s.renormalize()   # not a timed launch: the per-part times stay per part
     s.synchronize()
     if not timed:

@@ -5,9 +5,10 @@
 
 1. Data: random layered circuits (U3 + random-matching CZ, the benchmark
    generator) with n ~ U{20..30} qubits and depth ~ U{4..10}. Each runs under
-   two plans drawn from the candidate set (fused FP64 at k = 5..7, the part
-   kernel at k = 8..13 with greedy parts or k = 5..10 with natural-order
-   parts); every part's kernel time is recorded with HIP events.
+   two plans drawn from the candidate set (costmodel.candidate_specs: fused
+   FP64 / BF16-split unitaries, wide BF16-split parts, the moving-layout
+   chain, the register part kernel); every part's kernel time is recorded
+   with HIP events, and a chain's layout restore on its own.
 2. Training: the 3-layer MLP (64 hidden units, FP32) on log(ms per 2^30
    amplitudes), by libhsv's own forward/backward/Adam kernels; 10% held out.
 3. Evaluation: 32-qubit random layered circuits of depth ~ U{30..60}. The
@@ -28,8 +29,8 @@ import numpy as np
 
 from . import capi
 from .circuit import random_layered
-from .costmodel import (MLP, PARAMS_FILE, build_plan, candidate_specs, choose_plan, fallback_plan, ms_of, part_features,
-                        predict_plan_ms, run_plan, target_of)
+from .costmodel import (MLP, PARAMS_FILE, RESTORE_MS_PER_2_30, build_plan, candidate_specs, choose_plan, fallback_plan,
+                        ms_of, plan_features, predict_plan_ms, run_plan, target_of)
 
 
 def log(msg):
@@ -49,7 +50,7 @@ def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10), max_seco
         picks = [int(i) for i in rng.choice(len(specs), size=2, replace=False)]
         jobs.append((n, depth, int(rng.integers(1 << 30)), picks))
     jobs = [j for b in range(0, num, 100) for j in sorted(jobs[b:b + 100], key=lambda j: j[0])]
-    X, y, meta = [], [], []
+    X, y, meta, restore = [], [], [], []
     t0 = time.time()
     s = None
     for c, (n, depth, gseed, picks) in enumerate(jobs):
@@ -65,9 +66,11 @@ def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10), max_seco
             plan = build_plan(gates, n, specs[pi])
             s.reset()
             times = run_plan(s, plan)
-            for part, ms in zip(plan["parts"], times):
-                X.append(part_features(part, n, plan["mode"]))
+            for f, ms in zip(plan_features(plan, n), times):
+                X.append(f)
                 y.append(target_of(float(ms), n))
+            if plan["mode"] == "chain" and len(times) == len(plan["parts"]) + 1:
+                restore.append((n, float(times[-1])))
             meta.append({"n": n, "depth": depth, "mode": plan["mode"], "k": plan["k"],
                          "partitioner": plan["partitioner"], "parts": len(plan["parts"]),
                          "ms": float(times.sum())})
@@ -75,7 +78,14 @@ def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10), max_seco
             log(f"[collect] {c + 1}/{num} circuits (n={n}), {len(X)} parts, {time.time() - t0:.0f} s")
     if s is not None:
         s.close()
-    return np.array(X, np.float32), np.array(y, np.float32), meta
+    return np.array(X, np.float32), np.array(y, np.float32), meta, np.array(restore, np.float64).reshape(-1, 2)
+
+
+def restore_coefficient(restore: np.ndarray) -> float:
+    """Median layout-restore time of the chain plans, per 2^30 amplitudes."""
+    if not len(restore):
+        return RESTORE_MS_PER_2_30
+    return float(np.median(restore[:, 1] / 2.0 ** (restore[:, 0] - 30)))
 
 
 def evaluate(model: MLP, num: int, seed: int, n: int = 32):
@@ -95,9 +105,9 @@ def evaluate(model: MLP, num: int, seed: int, n: int = 32):
                 t0 = time.perf_counter()
                 times = run_plan(s, plan)
                 wall = time.perf_counter() - t0
-                Xp = np.stack([part_features(p, n, plan["mode"]) for p in plan["parts"]])
-                pred = np.array([ms_of(float(t), n) for t in model.predict(Xp)])
-                part_err.extend(list(np.abs(pred - times) / times))
+                pred = np.array([ms_of(float(t), n) for t in model.predict(plan_features(plan, n))])
+                tp = times[:len(plan["parts"])]
+                part_err.extend(list(np.abs(pred - tp) / tp))
                 res[name] = {"mode": plan["mode"], "k": plan["k"], "partitioner": plan["partitioner"],
                              "parts": len(plan["parts"]), "predicted_ms": plan["predicted_ms"],
                              "measured_kernel_ms": float(times.sum()), "wall_s": wall}
@@ -128,13 +138,14 @@ def main(argv=None) -> int:
         parts = [np.load(p) for p in args.data.split(",") if p]
         X = np.concatenate([d["X"] for d in parts])
         y = np.concatenate([d["y"] for d in parts])
+        restore = np.concatenate([d["restore"] for d in parts if "restore" in d.files] or [np.zeros((0, 2))])
         meta = []
         args.circuits = int(sum(int(d["circuits"]) for d in parts))
     else:
-        X, y, meta = collect(args.circuits, args.seed, max_seconds=args.max_seconds)
+        X, y, meta, restore = collect(args.circuits, args.seed, max_seconds=args.max_seconds)
         args.circuits = len(meta) // 2
         if args.data:
-            np.savez(args.data, X=X, y=y, circuits=args.circuits)
+            np.savez(args.data, X=X, y=y, circuits=args.circuits, restore=restore)
             log(f"[collect] saved {len(X)} samples to {args.data}")
         if args.stage == "collect":
             return 0
@@ -144,6 +155,9 @@ def main(argv=None) -> int:
     nval = max(1, len(X) // 10)
     val, tr = idx[:nval], idx[nval:]
     model = MLP(seed=7)
+    model.restore_ms_per_2_30 = restore_coefficient(restore)
+    log(f"[train] chain layout restore: {model.restore_ms_per_2_30:.2f} ms per 2^30 amplitudes "
+        f"({len(restore)} samples)")
     t0 = time.time()
     loss = model.train(X[tr], y[tr], epochs=args.epochs, batch=512, lr=2e-3)
     train_s = time.time() - t0
@@ -159,7 +173,8 @@ def main(argv=None) -> int:
     with open(os.path.join(args.out, "costmodel_report.json"), "w") as f:
         json.dump(report, f, indent=1)
     model.save(os.path.join(args.out, "costmodel_params.json"),
-               meta={"samples": len(X), "val_median_rel_err": val_rel})
+               meta={"samples": len(X), "val_median_rel_err": val_rel,
+                     "restore_ms_per_2_30": model.restore_ms_per_2_30})
     log(json.dumps({k: v for k, v in report.items() if k not in ("eval", "collect_plans")}))
     return 0
 

@@ -38,10 +38,13 @@ def test_features_and_candidates():
     f = part_features(parts[0], n, "part")
     assert f.shape == (N_FEATURES,) and np.all(np.isfinite(f))
     plans = candidate_plans(gates, n)
-    assert {p["mode"] for p in plans} == {"part", "fused64", "fusedbf16"}
-    assert {p["partitioner"] for p in plans} == {"greedy", "nat", "dfs", "closure"}
+    assert {p["mode"] for p in plans} == {"part", "fused64", "fusedbf16", "chain"}
+    assert {p["partitioner"] for p in plans} == {"greedy", "nat", "dfs", "closure", "chain"}
     for p in plans:
         check_parts(gates, p["parts"])
+        if p["mode"] == "chain":
+            f = part_features(p["parts"][1], n, "chain", p["lows"][1])
+            assert f[1] * 10 in (12, 13) and f[-1] > 0   # tile width, fused steps
     check_parts(gates, fallback_plan(gates, n)["parts"], 10)
 
 
@@ -109,4 +112,4 @@ def test_trained_model_beats_plain_partitioner():
         infid = 1 - abs(a.inner(b)) ** 2
     model.close()
     assert t_model < 0.8 * t_fb, (best["mode"], best["k"], best["partitioner"], t_model, t_fb)
-    assert infid <= (1e-12 if best["mode"] != "fusedbf16" else 1e-6)
+    assert infid <= (1e-12 if best["mode"] not in ("fusedbf16", "chain") else 1e-6)
