@@ -481,9 +481,11 @@ int lower_all(const hsv_gate* gates, int num_gates, std::vector<Lowered>* out, c
 // In-place qubit permutation (amplitude bit s moves to bit new_slot[s]) as
 // two involutions: on each cycle c_0 -> c_1 -> ... -> c_{m-1} of the slot
 // map, the reflections s1: c_k -> c_{-k} and s2: c_k -> c_{1-k} (indices
-// mod m) compose to s2(s1(c_k)) = c_{k+1}. Two swap passes, no second buffer.
+// mod m) compose to s2(s1(c_k)) = c_{k+1}. Two tiled swap passes, no second
+// buffer. Needs n >= 10 (a tile of up to 10 slots).
 static int permute_in_place(hsv_state* s, const int* new_slot) {
   const int n = s->n;
+  if (n < 10) return fail(HSV_ERR_LIMIT, "the in-place permutation needs at least 10 qubits");
   std::vector<int> s1(n), s2(n);
   for (int q = 0; q < n; ++q) s1[q] = s2[q] = q;
   std::vector<char> done(n, 0);
@@ -501,20 +503,53 @@ static int permute_in_place(hsv_state* s, const int* new_slot) {
     }
   }
   if (int rc = begin_timed(s)) return rc;
-  for (const std::vector<int>* sig : {&s1, &s2}) {
-    BitPairs bp{};
-    for (int q = 0; q < n; ++q)
-      if ((*sig)[q] > q) {
-        bp.lo[bp.npairs] = (uint8_t)q;
-        bp.hi[bp.npairs] = (uint8_t)(*sig)[q];
-        ++bp.npairs;
+  for (const std::vector<int>* sp : {&s1, &s2}) {
+    const std::vector<int>& sig = *sp;
+    bool any = false;
+    for (int q = 0; q < n; ++q) any = any || sig[q] != q;
+    if (!any) continue;
+    // the tile's slot set B: slots 0-4 and their partners, then more low
+    // slots (unpaired, or whole pairs) up to 9..10 slots
+    uint64_t B = 0;
+    for (int q = 0; q < 5; ++q) B |= (1ull << q) | (1ull << sig[q]);
+    for (int q = 5; q < n && __builtin_popcountll(B) < 10; ++q) {
+      if (B >> q & 1) continue;
+      const uint64_t add = (1ull << q) | (1ull << sig[q]);
+      if (__builtin_popcountll(B | add) <= 10) B |= add;
+    }
+    SwapTiles st{};
+    st.nb = __builtin_popcountll(B);
+    int bslot[10], pos[64];
+    for (int q = 0, j = 0; q < n; ++q)
+      if (B >> q & 1) {
+        bslot[j] = q;
+        pos[q] = j++;
       }
-    if (!bp.npairs) continue;
-    const int grid = grid_for(s->len, 256, uint64_t(s->num_cu) * 16);
+    for (int v = 0; v < 32; ++v)
+      for (int j = 0; j < 5; ++j) {
+        if (!(v >> j & 1)) continue;
+        st.lo[v] += 1ull << bslot[j];
+        st.slo[v] |= (uint16_t)(1u << pos[sig[bslot[j]]]);
+        if (5 + j < st.nb) {
+          st.hi[v] += 1ull << bslot[5 + j];
+          st.shi[v] |= (uint16_t)(1u << pos[sig[bslot[5 + j]]]);
+        }
+      }
+    const uint64_t all = (n == 64) ? ~0ull : ((1ull << n) - 1);
+    st.outer_mask = all & ~B;
+    for (int q = 0; q < n; ++q)
+      if (!(B >> q & 1) && sig[q] > q) {
+        st.pa[st.npairs] = (uint8_t)q;
+        st.pb[st.npairs] = (uint8_t)sig[q];
+        ++st.npairs;
+      }
+    const uint64_t nouter = 1ull << (n - st.nb);
+    if (n - st.nb > 28) return fail(HSV_ERR_LIMIT, "in-place permutation: more than 2^28 tiles");
+    const int grid = (int)std::min<uint64_t>(nouter, uint64_t(s->num_cu) * 8);
     if (s->dtype == HSV_C128)
-      hipLaunchKernelGGL((k_swap_bits<double>), grid, 256, 0, s->stream, amp_as<double>(s), s->len, bp);
+      hipLaunchKernelGGL((k_swap_tiles<double>), grid, kSwapThreads, 0, s->stream, amp_as<double>(s), nouter, st);
     else
-      hipLaunchKernelGGL((k_swap_bits<float>), grid, 256, 0, s->stream, amp_as<float>(s), s->len, bp);
+      hipLaunchKernelGGL((k_swap_tiles<float>), grid, kSwapThreads, 0, s->stream, amp_as<float>(s), nouter, st);
     HIPCHK(hipGetLastError());
   }
   return end_timed(s);

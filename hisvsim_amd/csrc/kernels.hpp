@@ -423,39 +423,76 @@ __global__ __launch_bounds__(256) void k_zero_state(typename VecT<R>::type* __re
     a[i] = cmk<V, R>(i == 0 ? R(1) : R(0), R(0));
 }
 
-// ---- qubit permutation (out of place) --------------------------------------
-// out[sigma(i)] = in[i]; sigma moves bit s to bit new_slot[s]. sigma(i) is
-// looked up per 9-bit chunk of i (4 chunks: n <= 36) from tables built per
-// workgroup in LDS (16 KB).
 // ---- in-place qubit involution ---------------------------------------------
-// The qubits of each pair (lo[p], hi[p]) trade slots: amplitudes a[i] and
-// a[sigma(i)] swap, sigma exchanging the bit values of every pair. Each
-// moved amplitude is read and written once; a pair is handled by the thread
-// of its lower index. Any qubit permutation is two such involutions.
-struct BitPairs {
-  int npairs;
-  uint8_t lo[32], hi[32];
+// The qubits of each slot pair trade slots: amplitudes a[i] and a[sigma(i)]
+// swap, sigma exchanging the two bit values of every pair. Any qubit
+// permutation is two such involutions. Tiled so that every access covers
+// 512-byte runs: a tile is the 2^nb amplitudes over the slot set B = slots
+// 0-4 plus their partners (nb <= 10); sigma maps the tile at outer address o
+// onto the tile at sigma_hi(o) (the pairs outside B act on o) with its
+// positions permuted by sigma_lo (the pairs inside B). A workgroup takes the
+// pair of tiles (o, sigma_hi(o)) with o <= sigma_hi(o): both are loaded
+// coalesced into LDS, and each position f of one is stored from position
+// sigma_lo(f) of the other. Exact (FP64 stays FP64); every amplitude is read
+// and written once.
+struct SwapTiles {
+  uint64_t outer_mask;           // slots outside B
+  uint64_t lo[32], hi[32];       // address of tile position bits 0-4 / 5-9
+  uint16_t slo[32], shi[32];     // sigma_lo of a tile position: slo[f & 31] | shi[f >> 5]
+  int nb;                        // |B|
+  int npairs;                    // pairs outside B
+  uint8_t pa[32], pb[32];
 };
+constexpr int kSwapThreads = 256;
 template <typename R>
-__global__ __launch_bounds__(256) void k_swap_bits(typename VecT<R>::type* __restrict__ a, uint64_t len,
-                                                   BitPairs bp) {
+__global__ __launch_bounds__(kSwapThreads) void k_swap_tiles(typename VecT<R>::type* __restrict__ a, uint64_t nouter,
+                                                           SwapTiles st) {
   using V = typename VecT<R>::type;
-  const uint64_t stride = uint64_t(gridDim.x) * 256;
-  for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < len; i += stride) {
-    uint64_t j = i;
-    for (int p = 0; p < bp.npairs; ++p) {
-      const uint64_t d = ((i >> bp.lo[p]) ^ (i >> bp.hi[p])) & 1;
-      j ^= (d << bp.lo[p]) | (d << bp.hi[p]);
+  __shared__ V ta[1024], tb[1024];
+  // outer address of tile w and of its partner, per 7-bit chunk of w
+  __shared__ uint64_t ot[4][128], ot2[4][128];
+  for (int i = threadIdx.x; i < 512; i += kSwapThreads) {
+    const int c = i >> 7, v = i & 127;
+    const uint64_t o = pdep64(uint64_t(v) << (7 * c), st.outer_mask);
+    uint64_t o2 = o;
+    for (int p = 0; p < st.npairs; ++p) {
+      const uint64_t d = ((o >> st.pa[p]) ^ (o >> st.pb[p])) & 1;
+      o2 ^= (d << st.pa[p]) | (d << st.pb[p]);
     }
-    if (j > i) {
-      const V x = ldg<true>(&a[i]);
-      const V y = ldg<true>(&a[j]);
-      stg<true>(&a[i], y);
-      stg<true>(&a[j], x);
+    ot[c][v] = o;
+    ot2[c][v] = o2;
+  }
+  const int T = 1 << st.nb;
+  for (uint64_t w = blockIdx.x; w < nouter; w += gridDim.x) {
+    const int w0 = w & 127, w1 = (w >> 7) & 127, w2 = (w >> 14) & 127, w3 = (w >> 21) & 127;
+    __syncthreads();  // tables built; the previous tile's LDS reads are done
+    const uint64_t o = ot[0][w0] | ot[1][w1] | ot[2][w2] | ot[3][w3];
+    const uint64_t o2 = ot2[0][w0] | ot2[1][w1] | ot2[2][w2] | ot2[3][w3];
+    if (o2 < o) continue;  // the partner tile's workgroup does this pair
+    const bool pair = o2 != o;
+    for (int f = threadIdx.x; f < T; f += kSwapThreads) {
+      const uint64_t off = st.lo[f & 31] + st.hi[f >> 5];
+      ta[f] = ldg<true>(&a[o + off]);
+      if (pair) tb[f] = ldg<true>(&a[o2 + off]);
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < T; f += kSwapThreads) {
+      const uint64_t off = st.lo[f & 31] + st.hi[f >> 5];
+      const int g = st.slo[f & 31] | st.shi[f >> 5];
+      if (pair) {
+        stg<true>(&a[o2 + off], ta[g]);
+        stg<true>(&a[o + off], tb[g]);
+      } else {
+        stg<true>(&a[o + off], ta[g]);
+      }
     }
   }
 }
 
+// ---- qubit permutation (out of place) --------------------------------------
+// out[sigma(i)] = in[i]; sigma moves bit s to bit new_slot[s]. sigma(i) is
+// looked up per 9-bit chunk of i (4 chunks: n <= 36) from tables built per
+// workgroup in LDS (16 KB).
 constexpr int kPermChunk = 9;
 template <typename R>
 __global__ __launch_bounds__(256) void k_permute(const typename VecT<R>::type* __restrict__ in,
