@@ -405,8 +405,8 @@ __global__ __launch_bounds__(kBfTileThreads) void k_fused_bft(typename VecT<R>::
   }
 }
 
-// ---- BF16-split wide parts: 12- or 13-qubit tiles, 5-qubit fused steps ------------
-// For a part of 8..13 qubits the host splits the gates into steps of at most
+// ---- BF16-split wide parts: 12- to 14-qubit tiles, 5-qubit fused steps ------------
+// For a part of 8..14 qubits the host splits the gates into steps of at most
 // 5 qubits (the qubit-set closure method), fuses each step into a 32 x 32
 // FP64 unitary and sends its BF16 A fragments. A workgroup owns one tile of
 // 2^KT amplitudes (the part's qubits, padded with the lowest free ones, in
@@ -427,23 +427,26 @@ __global__ __launch_bounds__(kBfTileThreads) void k_fused_bft(typename VecT<R>::
 // columns at a stride of 32+ elements: 16-way bank conflicts.
 struct PartBfStep {
   uint16_t rt[32];              // swizzled LDS offset of step row r
-  uint16_t ctl[16], cth[16];    // of step column c: ctl[c & 15] ^ cth[c >> 4]
+  uint16_t ctl[16], cth[32];    // of step column c: ctl[c & 15] ^ cth[c >> 4]
 };
 struct PartBfTables {
-  uint64_t alo[64], ahi[128];   // address offset of tile index bits 0-5 / 6-12
+  uint64_t alo[64], ahi[256];   // address offset of tile index bits 0-5 / 6-13
   // A layout change moves qubits between the tile's slots: the element
   // stored at tile position u is the one loaded at position plo[u & 63] |
   // phi[u >> 6] (the identity without a change), so the stores stay as
   // coalesced as the loads.
-  uint16_t plo[64], phi[128];
-  uint8_t swz[512];             // h(t >> 4): XOR into the low 5 bits of t
+  uint16_t plo[64], phi[256];
+  uint8_t swz[1024];            // h(t >> 4): XOR into the low 5 bits of t
 };
-// 512 threads: 8 amplitudes per thread for 12-qubit tiles, 16 for 13.
-// 12-qubit tiles keep every step's U fragments in LDS; 13-qubit tiles (64 KB
-// of FP32 in LDS) stream them through one 8 KB buffer, each thread holding
-// its 16-byte share of the next step in a register, so two workgroups fit a
-// CU and one computes while the other loads or stores.
-constexpr int part_bf_threads(int) { return 512; }
+// 512 threads: 8 amplitudes per thread for 12-qubit tiles, 16 for 13; 1024
+// threads (16 amplitudes each) for 14. 12-qubit tiles keep every step's U
+// fragments in LDS; 13-qubit tiles (64 KB of FP32 in LDS) stream them
+// through one 8 KB buffer, each of 512 threads holding its 16-byte share of
+// the next step in a register, so two workgroups fit a CU and one computes
+// while the other loads or stores. A 14-qubit tile (128 KB) fills the CU
+// alone, so its compute does not overlap its I/O (66 ms against 54 ms per
+// pass at 33 qubits): it saves passes only where parts get much wider.
+constexpr int part_bf_threads(int kt) { return kt >= 14 ? 1024 : 512; }
 constexpr int kPartBfMaxSteps = 12;
 
 template <typename R, int KT>
@@ -462,7 +465,7 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
   constexpr bool STREAM_U = KT >= 13;
   constexpr int STEP_FRAGS = 2 * 4 * 64;      // bf8 per step: [rb][set][lane]
   static_assert(CBW >= 1 && NCB % W == 0 && PER * NTH == T, "tile shape");
-  static_assert(!STREAM_U || STEP_FRAGS == NTH, "one fragment per thread per step");
+  static_assert(!STREAM_U || STEP_FRAGS <= NTH, "at most one fragment per thread per step");
   using V = typename VecT<R>::type;
   typedef float v4f_t __attribute__((ext_vector_type(4)));
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -470,8 +473,8 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
   bf8* su = STREAM_U ? su_one : reinterpret_cast<bf8*>(smem_raw);     // [step][rb][set][lane]
   __shared__ float2 ys[T];
   __shared__ PartBfStep st[kPartBfMaxSteps];
-  __shared__ uint64_t alo[64], ahi[128];
-  __shared__ uint16_t plo[64], phi[128];
+  __shared__ uint64_t alo[64], ahi[T >> 6];
+  __shared__ uint16_t plo[64], phi[T >> 6];
   __shared__ uint8_t swz[T >> 4];
   if constexpr (!STREAM_U)
     for (int i = threadIdx.x; i < nsteps * STEP_FRAGS; i += NTH) su[i] = ufrag[i];
@@ -481,9 +484,9 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
     for (int i = threadIdx.x; i < nsteps * (int)(sizeof(PartBfStep) / 4); i += NTH) dst[i] = src[i];
   }
   for (int i = threadIdx.x; i < 64; i += NTH) alo[i] = tabs->alo[i];
-  for (int i = threadIdx.x; i < 128; i += NTH) ahi[i] = tabs->ahi[i];
+  for (int i = threadIdx.x; i < (T >> 6); i += NTH) ahi[i] = tabs->ahi[i];
   for (int i = threadIdx.x; i < 64; i += NTH) plo[i] = tabs->plo[i];
-  for (int i = threadIdx.x; i < 128; i += NTH) phi[i] = tabs->phi[i];
+  for (int i = threadIdx.x; i < (T >> 6); i += NTH) phi[i] = tabs->phi[i];
   for (int i = threadIdx.x; i < (T >> 4); i += NTH) swz[i] = tabs->swz[i];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -497,7 +500,8 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
   if (t0 >= ntiles) return;  // uniform across the workgroup
   const int nt = (int)min(uint64_t(tiles_per_wg), ntiles - t0);
   bf8 fr;  // streamed fragments: this thread's share of the next step
-  if constexpr (STREAM_U) fr = ufrag[threadIdx.x];
+  if constexpr (STREAM_U)
+    if (threadIdx.x < STEP_FRAGS) fr = ufrag[threadIdx.x];
   __syncthreads();  // tables ready
   V raw[PER];
   if constexpr (!STREAM_U) {
@@ -536,8 +540,10 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
       const PartBfStep& S = st[s];
       const bf8* As;
       if constexpr (STREAM_U) {
-        su_one[threadIdx.x] = fr;  // every wave is past the previous step's reads
-        fr = ufrag[(s + 1 < nsteps ? s + 1 : 0) * STEP_FRAGS + threadIdx.x];
+        if (threadIdx.x < STEP_FRAGS) {
+          su_one[threadIdx.x] = fr;  // every wave is past the previous step's reads
+          fr = ufrag[(s + 1 < nsteps ? s + 1 : 0) * STEP_FRAGS + threadIdx.x];
+        }
         __syncthreads();
         As = su_one + lane;
       } else {
@@ -822,7 +828,7 @@ int rank_gf2(uint32_t* v, int n) {
 
 struct PartSwizzleKey { int cbit[4]; int rbit0; };
 
-void choose_part_swizzle(int KT, const std::vector<PartSwizzleKey>& keys, uint32_t nimg[13]) {
+void choose_part_swizzle(int KT, const std::vector<PartSwizzleKey>& keys, uint32_t nimg[14]) {
   auto score = [&](const uint32_t* m) {
     int sc = 0;
     for (const PartSwizzleKey& k : keys) {
@@ -835,12 +841,12 @@ void choose_part_swizzle(int KT, const std::vector<PartSwizzleKey>& keys, uint32
     return sc;
   };
   const int best_possible = 3 * (int)keys.size();
-  uint32_t cand[13] = {0};
+  uint32_t cand[14] = {0};
   std::memcpy(nimg, cand, sizeof(cand));
   int best = score(cand);
   uint64_t rng = 0x9E3779B97F4A7C15ull;
   for (int it = 0; it < 2000 && best < best_possible; ++it) {
-    for (int c = 0; c < 13; ++c) {
+    for (int c = 0; c < 14; ++c) {
       rng ^= rng << 13; rng ^= rng >> 7; rng ^= rng << 17;
       cand[c] = c < 4 ? 0u : (c == 4 ? uint32_t(rng & 15u) : (c < KT ? uint32_t(rng & 31u) : 0u));
     }
@@ -852,7 +858,7 @@ void choose_part_swizzle(int KT, const std::vector<PartSwizzleKey>& keys, uint32
 template <typename R, int KT>
 void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t steps_off, size_t frag_off,
                     int nsteps, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer, int tpw, int* rc) {
-  const size_t lds = KT >= 13 ? 0 : size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);  // 13: fragments streamed
+  const size_t lds = KT >= 13 ? 0 : size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);  // 13, 14: fragments streamed
   auto kern = &k_part_bf<R, KT>;
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)std::max<size_t>(lds, 1)) != hipSuccess) {
@@ -865,7 +871,7 @@ void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t
                      reinterpret_cast<const bf8*>(dev + frag_off), nsteps, ntiles, outer, tpw);
 }
 
-// A part of 8..13 qubits on the BF16-split path (see k_part_bf).
+// A part of 8..14 qubits on the BF16-split path (see k_part_bf).
 // tile_qubits: the (logical) qubits the tile must hold, at least those of
 // the gates. Logical qubit q sits in slot phys[q] (q when phys is null) and
 // the store leaves it in slot phys_after[q] (unchanged when null): a layout
@@ -888,9 +894,9 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   // 512-byte runs: a part that fits 12 qubits only without them takes a
   // 13-qubit tile.
   const uint64_t low5 = (1ull << std::min(n, 5)) - 1;
-  const int KT = __builtin_popcountll(pmask | low5) <= 12 ? 12 : 13;
-  if (K > 13) {
-    *err = "the bf16 fused path takes parts of at most 13 qubits";
+  const int KT = std::max(12, std::min(14, __builtin_popcountll(pmask | low5)));
+  if (K > 14) {
+    *err = "the bf16 fused path takes parts of at most 14 qubits";
     return HSV_ERR_LIMIT;
   }
   if (n < KT) {
@@ -899,7 +905,7 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   }
   uint64_t tmask = pmask;
   for (int b = 0; b < n && __builtin_popcountll(tmask) < KT; ++b) tmask |= 1ull << b;
-  int slot[13], sslot[13];
+  int slot[14], sslot[14];
   uint64_t smask = 0;
   for (int b = 0, j = 0; b < 64 && j < KT; ++b)
     if (tmask >> b & 1) {
@@ -948,7 +954,7 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     std::vector<Lowered> sg;
     for (int i : groups[s]) { gm |= qm[i]; sg.push_back(*gl[i]); }
     for (int b = 0; b < KT && __builtin_popcountll(gm) < 5; ++b) gm |= 1ull << b;  // pad: U (x) I
-    int rbit[5], cbit[8];
+    int rbit[5], cbit[9];
     std::vector<int> qubits;
     for (int b = 0, r = 0, c = 0; b < KT; ++b) {
       if (gm >> b & 1) { rbit[r++] = b; qubits.push_back(inv[slot[b]]); }
@@ -968,10 +974,10 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
       for (int b = 0; b < 5; ++b)
         if (r >> b & 1) S.rt[r] |= (uint16_t)(1u << rbit[b]);
     const int ncb = KT - 5;
-    for (int v = 0; v < 16; ++v)
-      for (int j = 0; j < 4; ++j) {
+    for (int v = 0; v < 32; ++v)
+      for (int j = 0; j < 5; ++j) {
         if (!(v >> j & 1)) continue;
-        if (j < ncb) S.ctl[v] |= (uint16_t)(1u << cbit[j]);
+        if (j < 4 && j < ncb && v < 16) S.ctl[v] |= (uint16_t)(1u << cbit[j]);
         if (4 + j < ncb) S.cth[v] |= (uint16_t)(1u << cbit[4 + j]);
       }
     for (int rb = 0; rb < 2; ++rb)
@@ -992,7 +998,7 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
         }
   }
   // swizzle the steps' LDS offsets (h is linear: XOR of the bits' images)
-  uint32_t nimg[13];
+  uint32_t nimg[14];
   choose_part_swizzle(KT, keys, nimg);
   auto hmap = [&](uint32_t t) {
     uint32_t h = 0;
@@ -1010,31 +1016,32 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   PartBfTables T0, T;
   std::memset(&T0, 0, sizeof(T0));
   for (int i = 0; i < (1 << (KT - 4)); ++i) T0.swz[i] = (uint8_t)hmap(uint32_t(i) << 4);
-  for (int v = 0; v < 128; ++v)
-    for (int j = 0; j < 7; ++j) {
+  for (int v = 0; v < 256; ++v)
+    for (int j = 0; j < 8; ++j) {
       if (!(v >> j & 1)) continue;
       if (v < 64 && j < 6) T0.alo[v] += 1ull << slot[j];
       if (6 + j < KT) T0.ahi[v] += 1ull << slot[6 + j];
     }
   for (int v = 0; v < 64; ++v) T0.plo[v] = (uint16_t)v;
-  for (int v = 0; v < 128; ++v) T0.phi[v] = (uint16_t)(v << 6);
+  for (int v = 0; v < 256; ++v) T0.phi[v] = (uint16_t)(v << 6);
   // tile bit j's qubit moves to tile bit sig[j] (the tile position of its new slot)
-  int sig_inv[13];
+  int sig_inv[14];
   for (int j = 0; j < KT; ++j)
     for (int k = 0; k < KT; ++k)
       if (slot[k] == sslot[j]) sig_inv[k] = j;
   T = T0;
   std::memset(T.plo, 0, sizeof(T.plo));
   std::memset(T.phi, 0, sizeof(T.phi));
-  for (int v = 0; v < 128; ++v)
-    for (int k = 0; k < 7; ++k) {
+  for (int v = 0; v < 256; ++v)
+    for (int k = 0; k < 8; ++k) {
       if (!(v >> k & 1)) continue;
       if (v < 64 && k < 6) T.plo[v] |= (uint16_t)(1u << sig_inv[k]);
       if (6 + k < KT) T.phi[v] |= (uint16_t)(1u << sig_inv[6 + k]);
     }
   // LDS: the tile (FP32) + step tables + tile tables statically, 8 KB per step
   const size_t static_lds =
-      (size_t(1) << KT) * 8 + sizeof(PartBfStep) * kPartBfMaxSteps + 192 * 8 + 192 * 2 + (size_t(1) << (KT - 4));
+      (size_t(1) << KT) * 8 + sizeof(PartBfStep) * kPartBfMaxSteps + (64 + (size_t(1) << (KT - 6))) * 10 +
+      (size_t(1) << (KT - 4));
   const int max_steps = KT >= 13 ? kPartBfMaxSteps
                                  : (int)std::min<size_t>(kPartBfMaxSteps, (160 * 1024 - static_lds) / (2 * 4 * 64 * 16));
   const size_t steps_off = 2 * sizeof(PartBfTables);
@@ -1067,8 +1074,10 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     const size_t fo = frag_off + size_t(s0) * 2 * 4 * 64 * 16;
     if (KT == 12) c128 ? launch_part_bf<double, 12>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
                        : launch_part_bf<float, 12>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
-    else c128 ? launch_part_bf<double, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
-              : launch_part_bf<float, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
+    else if (KT == 13) c128 ? launch_part_bf<double, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
+                            : launch_part_bf<float, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
+    else c128 ? launch_part_bf<double, 14>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
+              : launch_part_bf<float, 14>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
   }
   if (rc) { *err = "wide bf16 fused kernel attribute setup failed"; return rc; }
   const hipError_t e = hipGetLastError();

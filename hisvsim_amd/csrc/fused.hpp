@@ -23,9 +23,9 @@ struct BfTune {
   int variant = 2;   // parts of 5..7 qubits: 0 / 1 column-block kernel (1: four
                      // consecutive rows per column in each load), 2 tile kernel
   int tile_tpw = 8;  // tiles of 4096 amplitudes per workgroup, tile kernel
-  int part_tpw = 0;  // tiles per workgroup, wide-part kernel (8..13 qubits);
+  int part_tpw = 0;  // tiles per workgroup, wide-part kernel (8..14 qubits);
                      // 0: 8 for 12-qubit tiles (the next tile is prefetched
-                     // during the steps), 1 for 13-qubit tiles
+                     // during the steps), 1 for 13- and 14-qubit tiles
 };
 
 // Applies the fused unitary of `gates` to the n-qubit state at `amp`
@@ -35,7 +35,7 @@ struct BfTune {
 int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
                 const std::vector<Lowered>& gates, int mode, std::string* err, const BfTune& bf = BfTune());
 
-// One part on the BF16-split wide-part kernel (a 12- or 13-qubit tile that
+// One part on the BF16-split wide-part kernel (a 12- to 14-qubit tile that
 // holds slots 0-4) with a qubit layout change: logical qubit q sits in slot
 // phys[q] and the pass leaves it in slot phys_after[q]; only qubits of the
 // tile may move. tile_qubits: the logical qubits the tile must hold (at

@@ -815,7 +815,7 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
       if (phys[q] < 5) lowq |= 1ull << q;
     }
     if (!away) return HSV_OK;
-    if (__builtin_popcountll(away | lowq) <= 13) {
+    if (__builtin_popcountll(away | lowq) <= 14) {
       if (int rc = begin_timed(s)) return rc;
       std::string err;
       if (int rc = fused_part_bf_layout(s->amp, n, s->dtype, s->stream, &s->fused_scratch, {}, away | lowq,

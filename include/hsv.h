@@ -67,7 +67,7 @@ enum hsv_fused_mode {
   HSV_FUSED_F64 = 0,        /* FP64 matrix cores (v_mfma_f64)            */
   HSV_FUSED_F32SPLIT = 1,   /* FP32 matrix cores, hi/lo split products   */
   HSV_FUSED_BF16SPLIT = 2   /* BF16 matrix cores, hi/lo split products
-                               accumulated in FP32; parts of 5..7 qubits */
+                               accumulated in FP32; parts of 5..14 qubits */
 };
 
 /* Replaces StateVector(Qubit, int alloc) (state_vector.hpp:60-61, 74-119,
@@ -93,7 +93,10 @@ int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_s
 /* Not in the reference: the part's gates fused on the host into one
  * 2^k x 2^k FP64 unitary (checked for unitarity) and applied as a batched
  * complex GEMM on the matrix cores. k <= 7. HSV_FUSED_BF16SPLIT pads parts
- * to at least 5 qubits and needs at least 4 state qubits outside the part. */
+ * to at least 5 qubits and needs at least 4 state qubits outside the part;
+ * it also takes parts of 8..14 qubits, cut on the host into steps of at most
+ * 5 qubits (each fused and checked) that one pass applies to a 12- to
+ * 14-qubit tile holding qubits 0-4. */
 int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode);
 
 /* Not in the reference: a sequence of parts on the BF16-split path
@@ -102,13 +105,16 @@ int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode
  * gather_and_execute_on per part). gates holds the parts one after another,
  * part p being part_sizes[p] gates; lows[5p .. 5p+4] names the 5 qubits that
  * sit in slots 0-4 while part p runs (for p = 0: qubits 0-4, in any order).
- * Each part is one pass of a 12- or 13-qubit tile holding slots 0-4 (so its
+ * Each part is one pass of a 12- to 14-qubit tile holding slots 0-4 (so its
  * loads and stores cover 512-byte runs) and the part's qubits; the pass also
  * moves the next part's lows, which must be qubits of part p or of its lows,
  * into slots 0-4. So a part spends no tile slots on qubits it does not use
  * beyond those 5, and parts can be wider in useful qubits than with fixed
- * low qubits. Two in-place swap passes (the layout change as two
- * involutions) restore the standard layout at the end. Argument errors are
+ * low qubits. At the end, one tile pass (when the qubits still away from
+ * their slots and those in slots 0-4 fit 14) or two in-place swap passes
+ * (the layout change as two involutions) restore the standard layout. A
+ * tile pass rounds through FP32 like any pass of this path; the swap passes
+ * are exact. Argument errors are
  * found before any pass; a part that fails later (a gate on more than 5
  * qubits) leaves the parts before it applied, in the standard layout. */
 int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, const int* part_sizes,
@@ -191,9 +197,9 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *   "fused_tiles_per_wg" 4096-amplitude tiles each workgroup of the tile
  *                 form takes in turn (1..64, default 8);
  *   "part_tiles_per_wg"  the same for the wide BF16-split part kernel
- *                 (parts of 8..13 qubits; 0..64; 0, the default, = 8 for
+ *                 (parts of 8..14 qubits; 0..64; 0, the default, = 8 for
  *                 12-qubit tiles, whose next tile is prefetched during the
- *                 steps, and 1 for 13-qubit tiles). */
+ *                 steps, and 1 for 13- and 14-qubit tiles). */
 int hsv_tune(hsv_state* s, const char* key, int value);
 
 /* ---- Batched QAOA MaxCut (not in the reference) ------------------------

@@ -150,10 +150,10 @@ def test_fused_bf16split_infidelity(orc, k, seed, n, dtype):
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
 @pytest.mark.parametrize("k,n,ngates,seed", [(8, 14, 40, 1), (10, 16, 60, 2), (12, 16, 90, 3), (12, 20, 400, 4),
-                                             (13, 18, 120, 5), (13, 13, 60, 6)])
+                                             (13, 18, 120, 5), (13, 13, 60, 6), (14, 18, 150, 7), (14, 14, 80, 8)])
 def test_fused_bf16split_wide_parts(orc, k, n, ngates, seed, dtype):
-    """Parts of 8..13 qubits on the BF16-split path: split into fused steps
-    of <= 5 qubits applied to a 12- or 13-qubit tile in LDS (the 400-gate
+    """Parts of 8..14 qubits on the BF16-split path: split into fused steps
+    of <= 5 qubits applied to a 12-, 13- or 14-qubit tile in LDS (the 400-gate
     part needs more steps than fit at once and runs in several passes);
     infidelity and max error within 1e-6 of the FP64 oracle."""
     rng = np.random.default_rng(900 + seed)
@@ -192,7 +192,7 @@ def test_fused_bf16split_wide_part_off_the_low_qubits(orc, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
-@pytest.mark.parametrize("n,depth,k", [(14, 10, 12), (16, 12, 13), (20, 14, 13), (18, 8, 12)])
+@pytest.mark.parametrize("n,depth,k", [(14, 10, 12), (16, 12, 13), (20, 14, 13), (18, 8, 12), (20, 12, 14)])
 def test_fused_chain_matches_oracle(orc, n, depth, k, dtype):
     """Parts with a moving qubit layout (hsv_apply_fused_chain): each pass
     also brings the next part's lows into slots 0-4, and one permutation
