@@ -159,6 +159,24 @@ __global__ __launch_bounds__(4 * (1 << K)) void k_fused(typename VecT<R>::type* 
 // D[4*(l/16) + i][l%16].
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
+// Outer address of tile w: the tile index deposited into the slots outside
+// the tile. top holds the highest 0..3 outer slots; they vary fastest (the
+// lowest bits of w go to them, highest slot first), so the workgroups
+// running at one time spread over the halves / quarters of the index space:
+// measured on MI355X at 33 qubits, passes whose concurrent tiles sit in one
+// half of the state run slower.
+__device__ __forceinline__ uint64_t tile_base(uint64_t w, uint64_t outer_mask, uint64_t top) {
+  if (!top) return pdep64(w, outer_mask);
+  const int nt = __popcll(top);
+  uint64_t hi = 0, t = top;
+  for (int i = 0; i < nt; ++i) {  // bit i of w -> the (i+1)-th highest slot of top
+    const uint64_t b = 1ull << (63 - __clzll(t));
+    if (w >> i & 1) hi |= b;
+    t &= ~b;
+  }
+  return pdep64(w >> nt, outer_mask & ~top) | hi;
+}
+
 // Non-temporal 2-element loads and stores of an amplitude (one dwordx4 or
 // dwordx2 access with the streaming hint): the tile kernels touch each
 // amplitude once per pass.
@@ -304,7 +322,8 @@ template <typename R, int K>
 __global__ __launch_bounds__(kBfTileThreads) void k_fused_bft(typename VecT<R>::type* __restrict__ a,
                                                             const bf8* __restrict__ ufrag,
                                                             const BfTileTables* __restrict__ tabs,
-                                                            uint64_t ntiles, uint64_t outer_mask, int tiles_per_wg) {
+                                                            uint64_t ntiles, uint64_t outer_mask, uint64_t top,
+                                                            int tiles_per_wg) {
   constexpr int D = 1 << K;
   constexpr int RB = D / 16, KS = D / 32;
   constexpr int C = 4096 / D, NCB = C / 16;
@@ -338,7 +357,7 @@ __global__ __launch_bounds__(kBfTileThreads) void k_fused_bft(typename VecT<R>::
   for (int it = 0; it < tiles_per_wg; ++it) {
     const uint64_t tile = uint64_t(blockIdx.x) * tiles_per_wg + it;
     if (tile >= ntiles) break;  // uniform across the workgroup
-    const uint64_t base = pdep64(tile, outer_mask);
+    const uint64_t base = tile_base(tile, outer_mask, top);
     V raw[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
@@ -455,7 +474,8 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
                                                           const PartBfTables* __restrict__ tabs,
                                                           const PartBfStep* __restrict__ steps_g,
                                                           const bf8* __restrict__ ufrag, int nsteps,
-                                                          uint64_t ntiles, uint64_t outer_mask, int tiles_per_wg) {
+                                                          uint64_t ntiles, uint64_t outer_mask, uint64_t top,
+                                                          int tiles_per_wg) {
   constexpr int T = 1 << KT;
   constexpr int NCB = (1 << (KT - 5)) / 16;   // column blocks of a step
   constexpr int NTH = part_bf_threads(KT);
@@ -505,7 +525,7 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
   __syncthreads();  // tables ready
   V raw[PER];
   if constexpr (!STREAM_U) {
-    const uint64_t base = pdep64(t0, outer_mask);
+    const uint64_t base = tile_base(t0, outer_mask, top);
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int t = threadIdx.x + NTH * j;
@@ -513,7 +533,7 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
     }
   }
   for (int it = 0; it < nt; ++it) {
-    const uint64_t base = pdep64(t0 + it, outer_mask);
+    const uint64_t base = tile_base(t0 + it, outer_mask, top);
     if constexpr (STREAM_U) {
 #pragma unroll
       for (int j = 0; j < PER; ++j) {
@@ -528,7 +548,7 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
       ys[t ^ swz[t >> 4]] = make_float2((float)raw[j].x, (float)raw[j].y);
     }
     if (!STREAM_U && it + 1 < nt) {
-      const uint64_t nbase = pdep64(t0 + it + 1, outer_mask);
+      const uint64_t nbase = tile_base(t0 + it + 1, outer_mask, top);
 #pragma unroll
       for (int j = 0; j < PER; ++j) {
         const int t = threadIdx.x + NTH * j;
@@ -602,6 +622,13 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
 }
 
 namespace {
+
+// the highest `count` slots of an outer mask (the tile kernels' fast slots)
+uint64_t top_slots(uint64_t outer, int count) {
+  uint64_t top = 0;
+  for (int i = 0; i < count && (outer & ~top); ++i) top |= 1ull << (63 - __builtin_clzll(outer & ~top));
+  return top;
+}
 
 uint64_t pdep_host(uint64_t x, uint64_t mask) {
   uint64_t r = 0;
@@ -708,12 +735,13 @@ void launch_bf(void* amp, const void* scratch, int grid, hipStream_t st, const B
 // The BF16-split path: U as BF16 hi/lo fragments in the MFMA A-operand order,
 // columns in blocks of 16 along the lowest 4 slots outside the part.
 template <typename R, int K>
-void launch_bft(void* amp, const void* scratch, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer, int tpw) {
+void launch_bft(void* amp, const void* scratch, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer, uint64_t top,
+                int tpw) {
   const size_t frag_bytes = size_t(8) << (2 * K);
   const auto* U = static_cast<const bf8*>(scratch);
   const auto* T = reinterpret_cast<const BfTileTables*>(static_cast<const unsigned char*>(scratch) + frag_bytes);
   hipLaunchKernelGGL((k_fused_bft<R, K>), grid, kBfTileThreads, 0, st,
-                     static_cast<typename VecT<R>::type*>(amp), U, T, ntiles, outer, tpw);
+                     static_cast<typename VecT<R>::type*>(amp), U, T, ntiles, outer, top, tpw);
 }
 
 // Tables of the tile form: tile index bit j <-> the j-th lowest of the tile's
@@ -857,7 +885,8 @@ void choose_part_swizzle(int KT, const std::vector<PartSwizzleKey>& keys, uint32
 
 template <typename R, int KT>
 void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t steps_off, size_t frag_off,
-                    int nsteps, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer, int tpw, int* rc) {
+                    int nsteps, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer, uint64_t top, int tpw,
+                    int* rc) {
   const size_t lds = KT >= 13 ? 0 : size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);  // 13, 14: fragments streamed
   auto kern = &k_part_bf<R, KT>;
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -868,7 +897,7 @@ void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t
   hipLaunchKernelGGL(kern, grid, part_bf_threads(KT), lds, st, static_cast<typename VecT<R>::type*>(amp),
                      reinterpret_cast<const PartBfTables*>(dev + tabs_off),
                      reinterpret_cast<const PartBfStep*>(dev + steps_off),
-                     reinterpret_cast<const bf8*>(dev + frag_off), nsteps, ntiles, outer, tpw);
+                     reinterpret_cast<const bf8*>(dev + frag_off), nsteps, ntiles, outer, top, tpw);
 }
 
 // A part of 8..14 qubits on the BF16-split path (see k_part_bf).
@@ -878,7 +907,8 @@ void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t
 // change that must keep every qubit outside the tile where it is.
 int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
                   const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
-                  const int* phys_after, int tpw, std::string* err) {
+                  const int* phys_after, const BfTune& bf, std::string* err) {
+  int tpw = bf.part_tpw;
   int ph[64], pa[64], inv[64];
   for (int q = 0; q < n; ++q) ph[q] = phys ? phys[q] : q;
   for (int q = 0; q < n; ++q) {
@@ -997,6 +1027,15 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
           frags[(f + 3 * 64) * 8 + e] = il;
         }
   }
+  // tile bit j's qubit moves to tile bit sig[j] (the tile position of its new
+  // slot); store position u reads load position sig^-1(u)
+  int sig_inv[14];
+  for (int j = 0; j < KT; ++j)
+    for (int k = 0; k < KT; ++k)
+      if (slot[k] == sslot[j]) sig_inv[k] = j;
+  // the store gather of a layout change reads a quarter-wave's 16 positions
+  // along load bits sig^-1(0..3): its banks count like a step's columns
+  if (moves) keys.push_back(PartSwizzleKey{{sig_inv[0], sig_inv[1], sig_inv[2], sig_inv[3]}, sig_inv[4]});
   // swizzle the steps' LDS offsets (h is linear: XOR of the bits' images)
   uint32_t nimg[14];
   choose_part_swizzle(KT, keys, nimg);
@@ -1024,11 +1063,6 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     }
   for (int v = 0; v < 64; ++v) T0.plo[v] = (uint16_t)v;
   for (int v = 0; v < 256; ++v) T0.phi[v] = (uint16_t)(v << 6);
-  // tile bit j's qubit moves to tile bit sig[j] (the tile position of its new slot)
-  int sig_inv[14];
-  for (int j = 0; j < KT; ++j)
-    for (int k = 0; k < KT; ++k)
-      if (slot[k] == sslot[j]) sig_inv[k] = j;
   T = T0;
   std::memset(T.plo, 0, sizeof(T.plo));
   std::memset(T.phi, 0, sizeof(T.phi));
@@ -1062,6 +1096,7 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   const uint64_t ntiles = 1ull << (n - KT);
   if (tpw <= 0) tpw = KT == 12 ? 8 : 1;
   const int grid = (int)((ntiles + tpw - 1) / tpw);
+  const uint64_t top = top_slots(outer, bf.tile_order);
   const auto* dev = static_cast<const unsigned char*>(scratch->ptr);
   const bool c128 = dtype == HSV_C128;
   int rc = HSV_OK;
@@ -1072,12 +1107,12 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     const size_t to = s0 + cnt >= ns ? sizeof(PartBfTables) : 0;
     const size_t so = steps_off + sizeof(PartBfStep) * s0;
     const size_t fo = frag_off + size_t(s0) * 2 * 4 * 64 * 16;
-    if (KT == 12) c128 ? launch_part_bf<double, 12>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
-                       : launch_part_bf<float, 12>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
-    else if (KT == 13) c128 ? launch_part_bf<double, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
-                            : launch_part_bf<float, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
-    else c128 ? launch_part_bf<double, 14>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc)
-              : launch_part_bf<float, 14>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, tpw, &rc);
+    if (KT == 12) c128 ? launch_part_bf<double, 12>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+                       : launch_part_bf<float, 12>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
+    else if (KT == 13) c128 ? launch_part_bf<double, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+                            : launch_part_bf<float, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
+    else c128 ? launch_part_bf<double, 14>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+              : launch_part_bf<float, 14>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
   }
   if (rc) { *err = "wide bf16 fused kernel attribute setup failed"; return rc; }
   const hipError_t e = hipGetLastError();
@@ -1091,7 +1126,8 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
 // variant: 0 / 1 = column-block kernel (1: swizzled row order), 2 = tile form
 // (needs n >= 12; falls back to 1 below that).
 int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
-                   const std::vector<int>& Q, const std::vector<cd>& Um, int variant, int tpw, std::string* err) {
+                   const std::vector<int>& Q, const std::vector<cd>& Um, int variant, int tpw, int tile_order,
+                   std::string* err) {
   const int K = (int)Q.size();
   const int D = 1 << K;
   if (K < 5 || K > 7) {
@@ -1138,13 +1174,14 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
     const uint64_t ntiles = 1ull << (n - 12);
     const uint64_t outer = all & ~qmask & ~cmask;
     const int grid = (int)((ntiles + tpw - 1) / tpw);
+    const uint64_t top = top_slots(outer, tile_order);
     const bool c128 = dtype == HSV_C128;
-    if (K == 5) c128 ? launch_bft<double, 5>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw)
-                     : launch_bft<float, 5>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw);
-    else if (K == 6) c128 ? launch_bft<double, 6>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw)
-                          : launch_bft<float, 6>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw);
-    else c128 ? launch_bft<double, 7>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw)
-              : launch_bft<float, 7>(amp, scratch->ptr, grid, stream, ntiles, outer, tpw);
+    if (K == 5) c128 ? launch_bft<double, 5>(amp, scratch->ptr, grid, stream, ntiles, outer, top, tpw)
+                     : launch_bft<float, 5>(amp, scratch->ptr, grid, stream, ntiles, outer, top, tpw);
+    else if (K == 6) c128 ? launch_bft<double, 6>(amp, scratch->ptr, grid, stream, ntiles, outer, top, tpw)
+                          : launch_bft<float, 6>(amp, scratch->ptr, grid, stream, ntiles, outer, top, tpw);
+    else c128 ? launch_bft<double, 7>(amp, scratch->ptr, grid, stream, ntiles, outer, top, tpw)
+              : launch_bft<float, 7>(amp, scratch->ptr, grid, stream, ntiles, outer, top, tpw);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       *err = std::string("fused launch: ") + hipGetErrorString(e);
@@ -1194,7 +1231,7 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
 int fused_part_bf_layout(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
                          const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
                          const int* phys_after, const BfTune& bf, std::string* err) {
-  return part_apply_bf(amp, n, dtype, stream, scratch, gates, tile_qubits, phys, phys_after, bf.part_tpw, err);
+  return part_apply_bf(amp, n, dtype, stream, scratch, gates, tile_qubits, phys, phys_after, bf, err);
 }
 
 int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
@@ -1211,7 +1248,7 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
       for (int i = 0; i < L.nc; ++i) pmask |= 1ull << L.c[i];
     }
     if (__builtin_popcountll(pmask) > 7)
-      return part_apply_bf(amp, n, dtype, stream, scratch, gates, pmask, nullptr, nullptr, bf.part_tpw, err);
+      return part_apply_bf(amp, n, dtype, stream, scratch, gates, pmask, nullptr, nullptr, bf, err);
   }
   std::vector<int> Q;
   std::vector<cd> Um;
@@ -1229,7 +1266,8 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
     return HSV_ERR_LIMIT;
   }
   if (mode == HSV_FUSED_BF16SPLIT)
-    return fused_apply_bf(amp, n, dtype, stream, num_cu, scratch, Q, Um, bf.variant, std::max(bf.tile_tpw, 1), err);
+    return fused_apply_bf(amp, n, dtype, stream, num_cu, scratch, Q, Um, bf.variant, std::max(bf.tile_tpw, 1),
+                          bf.tile_order, err);
   int NC = 2048 / (16 * D);
   while (NC > 1 && 4 + __builtin_ctz(NC) > free_bits) NC >>= 1;
   const int C = 16 * NC;

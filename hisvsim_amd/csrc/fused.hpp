@@ -23,6 +23,8 @@ struct BfTune {
   int variant = 2;   // parts of 5..7 qubits: 0 / 1 column-block kernel (1: four
                      // consecutive rows per column in each load), 2 tile kernel
   int tile_tpw = 8;  // tiles of 4096 amplitudes per workgroup, tile kernel
+  int tile_order = 1;  // tile kernels: 1 = the highest slot outside the tile
+                       // varies fastest across workgroups, 0 = natural order
   int part_tpw = 0;  // tiles per workgroup, wide-part kernel (8..14 qubits);
                      // 0: 8 for 12-qubit tiles (the next tile is prefetched
                      // during the steps), 1 for 13- and 14-qubit tiles

@@ -199,7 +199,12 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *   "part_tiles_per_wg"  the same for the wide BF16-split part kernel
  *                 (parts of 8..14 qubits; 0..64; 0, the default, = 8 for
  *                 12-qubit tiles, whose next tile is prefetched during the
- *                 steps, and 1 for 13- and 14-qubit tiles). */
+ *                 steps, and 1 for 13- and 14-qubit tiles);
+ *   "tile_order"  order in which the BF16 tile kernels' workgroups take
+ *                 tiles: 1..3 = the highest 1..3 slots outside the tile vary
+ *                 fastest, so concurrent tiles spread over the halves
+ *                 (quarters, eighths) of the state (default 1; 2 and 3
+ *                 measured the same at 33 qubits); 0 = natural order. */
 int hsv_tune(hsv_state* s, const char* key, int value);
 
 /* ---- Batched QAOA MaxCut (not in the reference) ------------------------
