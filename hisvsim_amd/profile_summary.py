@@ -11,6 +11,10 @@
         HBM bytes per dispatch of kernel K: its FETCH_SIZE and WRITE_SIZE (KiB
         per dispatch) scaled by bytes / counter-KiB measured on kernel C, a
         copy kernel that reads and writes B bytes each
+    python -m hisvsim_amd.profile_summary steps DIR [--n 33 --depth 40 --k 13]
+        durations of the wide BF16-split part passes in a ``--kernel-trace``
+        run of the bench default, grouped by the 5-qubit fused steps of each
+        part (the bench's chain parts, regenerated here, in dispatch order)
 
 The CSV layouts are read by column name, so extra columns are ignored.
 """
@@ -90,6 +94,33 @@ def traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, kernel, cal_kern
             "fetch_bytes": fk * ff, "write_bytes": wk * wf, "bytes_per_launch": fk * ff + wk * wf}
 
 
+def trace_durations(d: str, kernel: str, min_ms: float = 0.0) -> List[float]:
+    """Per-dispatch durations (ms) of a kernel, in start order, from the
+    ``*_kernel_trace.csv`` under DIR."""
+    rows = []
+    for path in _csvs(d, "kernel_trace.csv"):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if kernel in r["Kernel_Name"]:
+                    ms = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+                    if ms >= min_ms:
+                        rows.append((int(r["Start_Timestamp"]), ms))
+    return [ms for _, ms in sorted(rows)]
+
+
+def steps_table(d: str, n: int, depth: int, k: int) -> dict:
+    from .circuit import random_layered
+    from .partition import chain_parts, closure_parts
+    parts, _ = chain_parts(random_layered(n, depth, seed=0), k)
+    durs = trace_durations(d, "k_part_bf", min_ms=1.0)[:len(parts)]
+    by: Dict[int, List[float]] = {}
+    for p, ms in zip(parts, durs):
+        by.setdefault(len(closure_parts(p, 5)), []).append(ms)
+    return {"parts": len(parts), "dispatches": len(durs),
+            "by_steps": {s: {"count": len(v), "mean_ms": statistics.mean(v), "min_ms": min(v), "max_ms": max(v)}
+                         for s, v in sorted(by.items())}}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -104,9 +135,16 @@ def main(argv=None) -> int:
     for k in ("fetch", "write", "cal_fetch", "cal_write", "kernel", "cal_kernel"):
         c.add_argument("--" + k.replace("_", "-"), required=True)
     c.add_argument("--cal-bytes", type=float, required=True)
+    e = sub.add_parser("steps")
+    e.add_argument("dir")
+    e.add_argument("--n", type=int, default=33)
+    e.add_argument("--depth", type=int, default=40)
+    e.add_argument("--k", type=int, default=13)
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(stats_text(args.dir, args.top))
+    elif args.cmd == "steps":
+        print(json.dumps(steps_table(args.dir, args.n, args.depth, args.k), indent=1))
     elif args.cmd == "counter":
         v = counter_values(args.dir, args.counter, args.kernel)
         print(json.dumps({"dispatches": len(v), "median": statistics.median(v) if v else None,
