@@ -569,6 +569,16 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
       } else {
         As = su + s * STEP_FRAGS + lane;
       }
+      // with several column blocks per wave, this step's U fragments are
+      // read once for all of them (the in-place writes below would otherwise
+      // force re-reads)
+      bf8 Af[2][4];
+      if constexpr (CBW > 1) {
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) Af[rb][q] = As[(rb * 4 + q) * 64];
+      }
 #pragma unroll
       for (int w = 0; w < CBW; ++w) {
         const int col = 16 * (wave * CBW + w) + cc;
@@ -585,8 +595,13 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
         }
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb) {
-          const bf8* A = As + rb * 4 * 64;
-          const bf8 urh = A[0], url = A[64], uih = A[128], uil = A[192];
+          bf8 urh, url, uih, uil;
+          if constexpr (CBW > 1) {
+            urh = Af[rb][0], url = Af[rb][1], uih = Af[rb][2], uil = Af[rb][3];
+          } else {
+            const bf8* A = As + rb * 4 * 64;
+            urh = A[0], url = A[64], uih = A[128], uil = A[192];
+          }
           v4f_t accR = {0, 0, 0, 0}, accT = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
           accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrh, accR, 0, 0, 0);
           accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrl, accR, 0, 0, 0);
