@@ -922,7 +922,7 @@ void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t
 // change that must keep every qubit outside the tile where it is.
 int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
                   const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
-                  const int* phys_after, const BfTune& bf, std::string* err) {
+                  const int* phys_after, uint64_t live_slots, const BfTune& bf, std::string* err) {
   int tpw = bf.part_tpw;
   int ph[64], pa[64], inv[64];
   for (int q = 0; q < n; ++q) ph[q] = phys ? phys[q] : q;
@@ -1107,8 +1107,10 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     return HSV_ERR_DEVICE;
   }
   const uint64_t all = (n == 64) ? ~0ull : ((1ull << n) - 1);
-  const uint64_t outer = all & ~tmask;
-  const uint64_t ntiles = 1ull << (n - KT);
+  // tiles with a set bit in a slot outside live_slots hold only zeros (and
+  // stay zero): they are skipped
+  const uint64_t outer = all & ~tmask & live_slots;
+  const uint64_t ntiles = 1ull << __builtin_popcountll(outer);
   if (tpw <= 0) tpw = KT == 12 ? 8 : 1;
   const int grid = (int)((ntiles + tpw - 1) / tpw);
   const uint64_t top = top_slots(outer, bf.tile_order);
@@ -1245,8 +1247,8 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
 
 int fused_part_bf_layout(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
                          const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
-                         const int* phys_after, const BfTune& bf, std::string* err) {
-  return part_apply_bf(amp, n, dtype, stream, scratch, gates, tile_qubits, phys, phys_after, bf, err);
+                         const int* phys_after, uint64_t live_slots, const BfTune& bf, std::string* err) {
+  return part_apply_bf(amp, n, dtype, stream, scratch, gates, tile_qubits, phys, phys_after, live_slots, bf, err);
 }
 
 int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
@@ -1263,7 +1265,7 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
       for (int i = 0; i < L.nc; ++i) pmask |= 1ull << L.c[i];
     }
     if (__builtin_popcountll(pmask) > 7)
-      return part_apply_bf(amp, n, dtype, stream, scratch, gates, pmask, nullptr, nullptr, bf, err);
+      return part_apply_bf(amp, n, dtype, stream, scratch, gates, pmask, nullptr, nullptr, ~0ull, bf, err);
   }
   std::vector<int> Q;
   std::vector<cd> Um;

@@ -41,10 +41,12 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
 // holds slots 0-4) with a qubit layout change: logical qubit q sits in slot
 // phys[q] and the pass leaves it in slot phys_after[q]; only qubits of the
 // tile may move. tile_qubits: the logical qubits the tile must hold (at
-// least those of the gates).
+// least those of the gates). live_slots: slots whose bit can be 1 in a
+// nonzero amplitude (the others hold |0> on the whole state, so tiles with
+// such a bit set are zero and are skipped); ~0 when nothing is known.
 int fused_part_bf_layout(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
                          const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
-                         const int* phys_after, const BfTune& bf, std::string* err);
+                         const int* phys_after, uint64_t live_slots, const BfTune& bf, std::string* err);
 
 // The fused unitary itself (for tests and the cost model): qubits of the
 // part (sorted, padded with the lowest free qubits to at least min_qubits)

@@ -73,6 +73,9 @@ struct hsv_state {
   // costs what the coalescing saves).
   int io_bits = 0;
   BfTune bf;                         // BF16-split fused kernel form and launch shape (see hsv_tune)
+  // The state is |0...0> (set by create and reset, cleared by every call
+  // that may change it): the chain then skips the tiles that are still zero.
+  bool basis0 = false;
   DeviceScratch fused_scratch;       // fused unitary + offset tables
   size_t elem() const { return dtype == HSV_C128 ? 16 : 8; }
 };
@@ -657,6 +660,7 @@ int hsv_reset(hsv_state* s) {
   else
     hipLaunchKernelGGL((k_zero_state<float>), grid, 256, 0, s->stream, amp_as<float>(s), s->len);
   HIPCHK(hipGetLastError());
+  s->basis0 = true;
   return HSV_OK;
 }
 
@@ -665,6 +669,7 @@ int hsv_apply_gate(hsv_state* s, const hsv_gate* g) {
   Lowered L;
   std::string err;
   if (int rc = lower(*g, &L, &err)) return fail(rc, err);
+  s->basis0 = false;
   if (int rc = begin_timed(s)) return rc;
   if (int rc = launch_gate(s, L)) return rc;
   g_gate_count++;
@@ -681,6 +686,7 @@ int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_s
     for (int i = 0; i < L.nc; ++i) part |= 1ull << L.c[i];
   }
   if (opt_slots) part |= (s->n >= 4 ? 0xFull : ((1ull << s->n) - 1));  // execute.hpp:184-199
+  s->basis0 = false;
   const int kmax = max_part_qubits(s);
   // Slots 0..2 make every gathered run at least 8 consecutive amplitudes; they
   // join the part whenever the LDS budget allows (results are unchanged).
@@ -706,6 +712,7 @@ int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode
   if (!s || (num_gates > 0 && !gates) || num_gates < 0) return fail(HSV_ERR_ARG, "bad arguments");
   std::vector<Lowered> Ls;
   if (int rc = lower_all(gates, num_gates, &Ls, s)) return rc;
+  s->basis0 = false;
   if (int rc = begin_timed(s)) return rc;
   std::string err;
   int rc = fused_apply(s->amp, s->n, s->dtype, s->stream, s->num_cu, &s->fused_scratch, Ls, mode, &err,
@@ -819,22 +826,36 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
       if (int rc = begin_timed(s)) return rc;
       std::string err;
       if (int rc = fused_part_bf_layout(s->amp, n, s->dtype, s->stream, &s->fused_scratch, {}, away | lowq,
-                                        phys.data(), ident.data(), s->bf, &err))
+                                        phys.data(), ident.data(), ~0ull, s->bf, &err))
         return fail(rc, "layout restore: " + err);
       return end_timed(s);
     }
     return permute_in_place(s, new_slot.data());
   };
+  // starting from |0...0>, a qubit no gate has touched yet is |0> everywhere:
+  // the passes skip the tiles with a 1 in its slot (the first parts run on a
+  // small corner of the state)
+  const uint64_t all_q = (n == 64) ? ~0ull : ((1ull << n) - 1);
+  uint64_t support = s->basis0 ? 0 : all_q;
+  s->basis0 = false;
   off = 0;
   for (int p = 0; p < num_parts; ++p) {
     const std::vector<Lowered> part(Ls.begin() + off, Ls.begin() + off + part_sizes[p]);
     off += part_sizes[p];
+    uint64_t live = 0;
+    for (int q = 0; q < n; ++q)
+      if (support >> q & 1) live |= 1ull << layout[p][q];
     if (int rc = begin_timed(s)) return rc;
     std::string err;
     if (int rc = fused_part_bf_layout(s->amp, n, s->dtype, s->stream, &s->fused_scratch, part, tile[p],
-                                      layout[p].data(), layout[p + 1].data(), s->bf, &err)) {
+                                      layout[p].data(), layout[p + 1].data(), live, s->bf, &err)) {
       restore(layout[p]);
       return fail(rc, "part " + std::to_string(p) + ": " + err);
+    }
+    for (const Lowered& L : part) {
+      if (L.kind == K_IDENT) continue;
+      for (int i = 0; i < L.nt; ++i) support |= 1ull << L.t[i];
+      for (int i = 0; i < L.nc; ++i) support |= 1ull << L.c[i];
     }
     if (int rc = end_timed(s)) return rc;
     g_part_count++;
@@ -845,6 +866,7 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
 
 int hsv_permute_qubits(hsv_state* s, const int* new_slot) {
   if (!s || !new_slot) return fail(HSV_ERR_ARG, "null argument");
+  s->basis0 = false;
   return permute_impl(s, new_slot);
 }
 
@@ -871,6 +893,7 @@ int hsv_write(hsv_state* s, uint64_t offset, uint64_t count, const double* host_
   if (!s || (count && !host_in) || offset > s->len || count > s->len - offset)
     return fail(HSV_ERR_ARG, "write range out of bounds");
   if (!count) return HSV_OK;
+  s->basis0 = false;
   if (s->dtype == HSV_C128) {
     HIPCHK(hipMemcpyAsync(static_cast<char*>(s->amp) + offset * 16, host_in, count * 16, hipMemcpyHostToDevice,
                           s->stream));
@@ -915,6 +938,7 @@ int hsv_probability(hsv_state* s, int qubit, int value, double* out) {
 
 int hsv_renormalize(hsv_state* s, double* norm2_before) {
   if (!s) return fail(HSV_ERR_ARG, "null handle");
+  s->basis0 = false;
   double n2 = 0;
   if (int rc = reduce<RD_NORM2>(s, nullptr, 0, &n2, nullptr)) return rc;
   if (norm2_before) *norm2_before = n2;
@@ -934,6 +958,7 @@ int hsv_project(hsv_state* s, uint64_t qubit_mask, uint64_t values, double* p_ou
   if (!s) return fail(HSV_ERR_ARG, "null handle");
   if (qubit_mask >> s->n) return fail(HSV_ERR_ARG, "qubit mask names a qubit beyond the state");
   if (values & ~qubit_mask) return fail(HSV_ERR_ARG, "values has bits outside qubit_mask");
+  s->basis0 = false;
   const int blocks = kRedBlocks;
   if (s->dtype == HSV_C128)
     hipLaunchKernelGGL((k_mass<double>), blocks, kRedBlock, 0, s->stream, amp_as<double>(s), s->len, qubit_mask,

@@ -209,6 +209,36 @@ def test_fused_chain_matches_oracle(orc, n, depth, k, dtype):
     assert np.abs(got - want).max() <= 1e-6
 
 
+@pytest.mark.parametrize("prep", ["reset", "reset+x", "write"])
+def test_fused_chain_from_basis_state(orc, prep):
+    """From |0...0> (after reset) the chain's first parts skip the tiles that
+    are still zero; any other call in between (a gate, a write) must turn
+    that off. Checked against the oracle from the same start."""
+    from hisvsim_amd.partition import chain_parts
+    n = 18
+    gates = random_layered(n, 10, seed=21)
+    parts, lows = chain_parts(gates, 13)
+    top = n - 1   # the first part does not touch every qubit
+    assert top not in {q for g in parts[0] for q in g.qubits()}
+    want = np.zeros(2 ** n, complex)
+    want[0] = 1
+    if prep == "reset+x":
+        orc.apply_gate(want, Gate("X", (top,)))
+    orc.apply_gates(want, gates)
+    with capi.State(n) as s:
+        if prep == "write":
+            s.write(np.eye(1, 2 ** n, 0, dtype=complex)[0])
+        else:
+            s.reset()
+        if prep == "reset+x":
+            s.apply_gate(Gate("X", (top,)))
+        s.apply_fused_chain(parts, lows)
+        s.renormalize()
+        got = s.read()
+    assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
+    assert np.abs(got - want).max() <= 1e-6
+
+
 def test_fused_chain_rejects_bad_lows(orc):
     """Argument errors are found before any pass: the state is unchanged."""
     from hisvsim_amd.partition import chain_parts
