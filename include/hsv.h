@@ -116,7 +116,10 @@ int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode
  * tile pass rounds through FP32 like any pass of this path; the swap passes
  * are exact. Argument errors are
  * found before any pass; a part that fails later (a gate on more than 5
- * qubits) leaves the parts before it applied, in the standard layout. */
+ * qubits) leaves the parts before it applied, in the standard layout.
+ * Right after hsv_create or hsv_reset (state |0...0>), a qubit that no
+ * earlier part has touched is |0> everywhere, so the first passes skip the
+ * tiles with a 1 in such a qubit. */
 int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, const int* part_sizes,
                           int num_parts, const int* lows, int mode);
 
