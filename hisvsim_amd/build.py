@@ -27,7 +27,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["hsv.hip", "fused.hip", "qaoa.hip", "mlp.hip"]
-CXX_SOURCES = ["gates.cpp"]
+CXX_SOURCES = ["gates.cpp", "partition.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -89,6 +89,9 @@ SIGNATURES = {
     "hsv_apply_part": (ctypes.c_int, [_P, _G, ctypes.c_int, ctypes.c_int]),
     "hsv_apply_fused": (ctypes.c_int, [_P, _G, ctypes.c_int, ctypes.c_int]),
     "hsv_apply_fused_chain": (ctypes.c_int, [_P, _G, ctypes.c_int, _I, ctypes.c_int, _I, ctypes.c_int]),
+    "hsv_chain_parts": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32), _I]),
     "hsv_permute_qubits": (ctypes.c_int, [_P, _I]),
     "hsv_read": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
     "hsv_write": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
@@ -186,6 +189,21 @@ def gate_matrix(g) -> np.ndarray:
     m = (out[0::2] + 1j * out[1::2])
     k = dim.value
     return m[:k * k].reshape(k, k)
+
+
+def chain_parts(gates: Sequence, k: int, low: int = 5, lookahead: int = 7, candidates: int = 24):
+    """hsv_chain_parts (host only): (parts, lows) as partition.chain_parts."""
+    masks = np.array([sum(1 << q for q in set(g.qubits())) for g in gates], dtype=np.uint64)
+    part = np.zeros(max(1, len(gates)), np.int32)
+    lows = np.zeros(max(1, len(gates)) * low + low, np.int32)
+    num = ctypes.c_int(0)
+    _check(lib().hsv_chain_parts(masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(gates), k, low,
+                                 lookahead, candidates, part.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                 lows.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(num)))
+    parts = [[] for _ in range(num.value)]
+    for g, pi in zip(gates, part[:len(gates)]):
+        parts[pi].append(g)
+    return parts, [list(map(int, lows[p * low:(p + 1) * low])) for p in range(num.value)]
 
 
 def fused_unitary(n: int, gates: Sequence):

@@ -317,16 +317,29 @@ def _grow(qsets: Sequence[frozenset], remaining: Sequence[int], base: set, k: in
     return Q
 
 
-def chain_parts(gates: Sequence[Gate], k: int, low: int = 5, lookahead: int = 7,
-                candidates: int = 24) -> Tuple[List[List[Gate]], List[List[int]]]:
+def chain_parts(gates: Sequence[Gate], k: int, low: int = 5, lookahead: int = 13,
+                candidates: int = 24, native: bool = True) -> Tuple[List[List[Gate]], List[List[int]]]:
     """Parts for a moving qubit layout (hsv_apply_fused_chain): part p is
     grown by the ``closure_parts`` method from the ``low`` qubits that sit in
     the lowest slots while it runs (lows[p]; lows[0] = qubits 0..low-1), so
     its tile - its qubits and lows[p] - has at most k qubits. lows[p + 1] is
     chosen among part p's tile: of the ``lookahead`` qubits whose next gates
-    come first, the ``low`` that let the next part absorb the most gates.
-    Unlike fixed low qubits, the low slots then hold qubits the next part
-    uses. Returns (parts, lows); the parts are acyclic as for closure_parts."""
+    come first, the ``low`` that let the next part absorb the most gates
+    (lookahead 13 tries every 5-subset of a 13-qubit tile). Unlike fixed low
+    qubits, the low slots then hold qubits the next part uses. Returns
+    (parts, lows); the parts are acyclic as for closure_parts. With
+    ``native`` the same method runs in libhsv (hsv_chain_parts, ~80x faster,
+    identical result) when the library loads."""
+    if native:
+        try:
+            from . import capi
+            capi.lib()
+        except Exception:   # no library: the Python method below
+            native = False
+        else:
+            if low > k:
+                raise ValueError("more low qubits than the part width")
+            return capi.chain_parts(gates, k, low, max(lookahead, low), candidates)
     from itertools import combinations
     if low > k:
         raise ValueError("more low qubits than the part width")

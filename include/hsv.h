@@ -177,6 +177,17 @@ int hsv_expect_pauli(hsv_state* s, uint64_t x_mask, uint64_t z_mask, double* out
  * gates on an n-qubit state: qubits_out (<= 7 entries, sorted) and U row-major
  * interleaved into u_out (2 * 4^7 doubles); returns its unitarity error. */
 int hsv_gate_matrix(const hsv_gate* g, double* out, int* dim);
+/* Host-only, not in the reference (in the role of its external dagP
+ * partitioner): the parts and low-slot sets for hsv_apply_fused_chain.
+ * gate_masks[i] = the qubits of gate i as a bit mask. Each part grows by the
+ * qubit-set closure from the `low` qubits in the low slots (at most k qubits
+ * with them); the next part's lows are the `low` of the `lookahead` qubits of
+ * the part's tile whose next gates come first that let the next part absorb
+ * the most gates. Out: part_of_gate[i] (parts keep the gate order),
+ * lows_out[p*low + j] ascending, *num_parts. HSV_ERR_LIMIT: a gate does not
+ * fit. Equals hisvsim_amd.partition.chain_parts gate for gate. */
+int hsv_chain_parts(const uint64_t* gate_masks, int num_gates, int k, int low, int lookahead, int candidates,
+                    int32_t* part_of_gate, int32_t* lows_out, int* num_parts);
 int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int* qubits_out,
                       int* num_part_qubits, double* u_out, double* unitarity_err);
 

@@ -208,6 +208,17 @@ def test_chain_parts_valid(n, depth, k):
     assert len(parts) <= len(closure_parts(gates, k, always=range(5)))
 
 
+@pytest.mark.parametrize("n,depth,k,lookahead", [(14, 8, 10, 7), (18, 10, 12, 13), (22, 8, 13, 9)])
+def test_chain_parts_native_equals_python(n, depth, k, lookahead):
+    """hsv_chain_parts (libhsv, host only) gives the Python method's parts and
+    lows gate for gate."""
+    gates = random_layered(n, depth, seed=3 * n)
+    py = chain_parts(gates, k, lookahead=lookahead, native=False)
+    nat = chain_parts(gates, k, lookahead=lookahead, native=True)
+    assert [[id(g) for g in p] for p in py[0]] == [[id(g) for g in p] for p in nat[0]]
+    assert py[1] == nat[1]
+
+
 def test_chain_parts_apply_like_the_circuit(orc):
     """Applying the chain's parts in order equals the circuit (oracle, n=12)."""
     gates = random_layered(12, 8, seed=4)
