@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libhsv.so")
 
 HSV_C128, HSV_C64 = 0, 1
 HSV_FUSED_F64, HSV_FUSED_F32SPLIT, HSV_FUSED_BF16SPLIT = 0, 1, 2
+CHAIN_RENORMALIZE = 0x100   # HSV_CHAIN_RENORMALIZE
 FUSED_MODES = {"f64": HSV_FUSED_F64, "f32split": HSV_FUSED_F32SPLIT, "bf16split": HSV_FUSED_BF16SPLIT}
 
 # SvSim::GateType numbering (basic_gates.hpp:12-45), as in hsv.h.
@@ -269,10 +270,12 @@ class State:
         _check(lib().hsv_apply_fused(self._h, arr, len(gates), m))
 
     def apply_fused_chain(self, parts: Sequence[Sequence], lows: Sequence[Sequence[int]],
-                          mode: str = "bf16split") -> None:
+                          mode: str = "bf16split", renormalize: bool = False) -> None:
         """Parts in sequence with a moving qubit layout (hsv_apply_fused_chain):
         lows[p] are the 5 qubits in slots 0-4 while part p runs (lows[0] =
-        qubits 0-4); each must be a qubit of the previous part or its lows."""
+        qubits 0-4); each must be a qubit of the previous part or its lows.
+        renormalize: end with the state scaled to unit norm
+        (HSV_CHAIN_RENORMALIZE)."""
         if len(lows) != len(parts):
             raise ValueError("one lows entry per part")
         flat = [g for p in parts for g in p]
@@ -282,7 +285,8 @@ class State:
         if any(len(low) != 5 for low in lows):
             raise ValueError("lows: 5 qubits per part")
         larr = (ctypes.c_int * max(1, len(lw)))(*lw)
-        _check(lib().hsv_apply_fused_chain(self._h, arr, len(flat), sizes, len(parts), larr, FUSED_MODES[mode]))
+        _check(lib().hsv_apply_fused_chain(self._h, arr, len(flat), sizes, len(parts), larr,
+                                           FUSED_MODES[mode] | (CHAIN_RENORMALIZE if renormalize else 0)))
 
     def permute_qubits(self, new_slot: Sequence[int]) -> None:
         arr = (ctypes.c_int * self.n)(*[int(x) for x in new_slot])

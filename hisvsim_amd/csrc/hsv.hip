@@ -486,8 +486,11 @@ int lower_all(const hsv_gate* gates, int num_gates, std::vector<Lowered>* out, c
 // map, the reflections s1: c_k -> c_{-k} and s2: c_k -> c_{1-k} (indices
 // mod m) compose to s2(s1(c_k)) = c_{k+1}. Two tiled swap passes, no second
 // buffer. Needs n >= 10 (a tile of up to 10 slots).
-static int permute_in_place(hsv_state* s, const int* new_slot) {
+// With renorm, the first pass also sums |a|^2 and the second scales by
+// 1 / norm (both passes must move data; *renormalized tells whether it was done).
+static int permute_in_place(hsv_state* s, const int* new_slot, bool renorm = false, bool* renormalized = nullptr) {
   const int n = s->n;
+  if (renormalized) *renormalized = false;
   if (n < 10) return fail(HSV_ERR_LIMIT, "the in-place permutation needs at least 10 qubits");
   std::vector<int> s1(n), s2(n);
   for (int q = 0; q < n; ++q) s1[q] = s2[q] = q;
@@ -505,12 +508,19 @@ static int permute_in_place(hsv_state* s, const int* new_slot) {
       s2[cyc[k]] = cyc[(1 - k + m) % m];
     }
   }
+  bool moves1 = false, moves2 = false;
+  for (int q = 0; q < n; ++q) {
+    moves1 = moves1 || s1[q] != q;
+    moves2 = moves2 || s2[q] != q;
+  }
+  renorm = renorm && moves1 && moves2;
   if (int rc = begin_timed(s)) return rc;
   for (const std::vector<int>* sp : {&s1, &s2}) {
     const std::vector<int>& sig = *sp;
     bool any = false;
     for (int q = 0; q < n; ++q) any = any || sig[q] != q;
     if (!any) continue;
+    const bool first = sp == &s1;
     // the tile's slot set B: slots 0-4 and their partners, then more low
     // slots (unpaired, or whole pairs) up to 9..10 slots
     uint64_t B = 0;
@@ -548,13 +558,23 @@ static int permute_in_place(hsv_state* s, const int* new_slot) {
       }
     const uint64_t nouter = 1ull << (n - st.nb);
     if (n - st.nb > 28) return fail(HSV_ERR_LIMIT, "in-place permutation: more than 2^28 tiles");
-    const int grid = (int)std::min<uint64_t>(nouter, uint64_t(s->num_cu) * 8);
+    // partials: s->partial[0 .. grid), then norm^2 and the factor
+    const int grid = (int)std::min<uint64_t>(nouter, std::min<uint64_t>(uint64_t(s->num_cu) * 8, 2 * kRedBlocks - 2));
+    double* norm_out = renorm && first ? s->partial : nullptr;
+    const double* scale = renorm && !first ? s->partial + 2 * kRedBlocks - 1 : nullptr;
     if (s->dtype == HSV_C128)
-      hipLaunchKernelGGL((k_swap_tiles<double>), grid, kSwapThreads, 0, s->stream, amp_as<double>(s), nouter, st);
+      hipLaunchKernelGGL((k_swap_tiles<double>), grid, kSwapThreads, 0, s->stream, amp_as<double>(s), nouter, st,
+                         norm_out, scale);
     else
-      hipLaunchKernelGGL((k_swap_tiles<float>), grid, kSwapThreads, 0, s->stream, amp_as<float>(s), nouter, st);
+      hipLaunchKernelGGL((k_swap_tiles<float>), grid, kSwapThreads, 0, s->stream, amp_as<float>(s), nouter, st,
+                         norm_out, scale);
     HIPCHK(hipGetLastError());
+    if (norm_out) {
+      hipLaunchKernelGGL(k_norm_scale, 1, 64, 0, s->stream, s->partial, grid, s->partial + 2 * kRedBlocks - 2);
+      HIPCHK(hipGetLastError());
+    }
   }
+  if (renormalized) *renormalized = renorm;
   return end_timed(s);
 }
 
@@ -728,6 +748,8 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
   if (!s || num_gates < 0 || num_parts < 0 || (num_gates > 0 && !gates) ||
       (num_parts > 0 && (!part_sizes || !lows)))
     return fail(HSV_ERR_ARG, "bad arguments");
+  const bool renorm = (mode & HSV_CHAIN_RENORMALIZE) != 0;
+  mode &= ~HSV_CHAIN_RENORMALIZE;
   if (mode != HSV_FUSED_BF16SPLIT) return fail(HSV_ERR_ARG, "the part chain runs on the BF16-split path only");
   const int n = s->n;
   if (n < 12) return fail(HSV_ERR_LIMIT, "the part chain needs a state of at least 12 qubits");
@@ -812,7 +834,7 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
   // back to the standard layout: the bit in slot phys[q] moves to slot q;
   // one tile pass without gates when the qubits still away from home and
   // those in slots 0-4 fit a tile, else two in-place swap passes
-  auto restore = [&](const std::vector<int>& phys) -> int {
+  auto restore = [&](const std::vector<int>& phys, bool with_renorm) -> int {
     std::vector<int> new_slot(n), ident(n);
     uint64_t away = 0, lowq = 0;
     for (int q = 0; q < n; ++q) {
@@ -821,16 +843,25 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
       if (phys[q] != q) away |= 1ull << q;
       if (phys[q] < 5) lowq |= 1ull << q;
     }
-    if (!away) return HSV_OK;
-    if (__builtin_popcountll(away | lowq) <= 14) {
+    bool done = false;
+    if (!away) {
+    } else if (__builtin_popcountll(away | lowq) <= 14) {
       if (int rc = begin_timed(s)) return rc;
       std::string err;
       if (int rc = fused_part_bf_layout(s->amp, n, s->dtype, s->stream, &s->fused_scratch, {}, away | lowq,
                                         phys.data(), ident.data(), ~0ull, s->bf, &err))
         return fail(rc, "layout restore: " + err);
-      return end_timed(s);
+      if (int rc = end_timed(s)) return rc;
+    } else if (int rc = permute_in_place(s, new_slot.data(), with_renorm, &done)) {
+      return rc;
     }
-    return permute_in_place(s, new_slot.data());
+    // the renormalisation, when the swap passes could not carry it
+    if (with_renorm && !done) {
+      const int rc = hsv_renormalize(s, nullptr);
+      if (rc == HSV_ERR_ARG) return HSV_OK;   // a zero state stays as it is
+      return rc;
+    }
+    return HSV_OK;
   };
   // starting from |0...0>, a qubit no gate has touched yet is |0> everywhere:
   // the passes skip the tiles with a 1 in its slot (the first parts run on a
@@ -849,7 +880,7 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
     std::string err;
     if (int rc = fused_part_bf_layout(s->amp, n, s->dtype, s->stream, &s->fused_scratch, part, tile[p],
                                       layout[p].data(), layout[p + 1].data(), live, s->bf, &err)) {
-      restore(layout[p]);
+      restore(layout[p], false);
       return fail(rc, "part " + std::to_string(p) + ": " + err);
     }
     for (const Lowered& L : part) {
@@ -861,7 +892,7 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
     g_part_count++;
     g_gate_count += part_sizes[p];
   }
-  return restore(layout[num_parts]);
+  return restore(layout[num_parts], renorm);
 }
 
 int hsv_permute_qubits(hsv_state* s, const int* new_slot) {

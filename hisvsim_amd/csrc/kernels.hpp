@@ -444,11 +444,18 @@ struct SwapTiles {
   uint8_t pa[32], pb[32];
 };
 constexpr int kSwapThreads = 256;
+// Optionally fused with a renormalisation: norm_out (one FP64 partial of
+// sum |a|^2 per workgroup, in a fixed order) in one pass, scale (a device
+// value, multiplied into every stored amplitude) in the next.
 template <typename R>
 __global__ __launch_bounds__(kSwapThreads) void k_swap_tiles(typename VecT<R>::type* __restrict__ a, uint64_t nouter,
-                                                           SwapTiles st) {
+                                                           SwapTiles st, double* __restrict__ norm_out,
+                                                           const double* __restrict__ scale) {
   using V = typename VecT<R>::type;
   __shared__ V ta[1024], tb[1024];
+  __shared__ double red[kSwapThreads / 64];
+  double acc = 0.0;
+  const R sc = scale ? R(*scale) : R(1);
   // outer address of tile w and of its partner, per 7-bit chunk of w
   __shared__ uint64_t ot[4][128], ot2[4][128];
   for (int i = threadIdx.x; i < 512; i += kSwapThreads) {
@@ -472,20 +479,54 @@ __global__ __launch_bounds__(kSwapThreads) void k_swap_tiles(typename VecT<R>::t
     const bool pair = o2 != o;
     for (int f = threadIdx.x; f < T; f += kSwapThreads) {
       const uint64_t off = st.lo[f & 31] + st.hi[f >> 5];
-      ta[f] = ldg<true>(&a[o + off]);
-      if (pair) tb[f] = ldg<true>(&a[o2 + off]);
+      const V x = ldg<true>(&a[o + off]);
+      ta[f] = x;
+      if (norm_out) acc += double(x.x) * double(x.x) + double(x.y) * double(x.y);
+      if (pair) {
+        const V y = ldg<true>(&a[o2 + off]);
+        tb[f] = y;
+        if (norm_out) acc += double(y.x) * double(y.x) + double(y.y) * double(y.y);
+      }
     }
     __syncthreads();
     for (int f = threadIdx.x; f < T; f += kSwapThreads) {
       const uint64_t off = st.lo[f & 31] + st.hi[f >> 5];
       const int g = st.slo[f & 31] | st.shi[f >> 5];
+      V x = ta[g];
+      x.x *= sc;
+      x.y *= sc;
       if (pair) {
-        stg<true>(&a[o2 + off], ta[g]);
-        stg<true>(&a[o + off], tb[g]);
+        V y = tb[g];
+        y.x *= sc;
+        y.y *= sc;
+        stg<true>(&a[o2 + off], x);
+        stg<true>(&a[o + off], y);
       } else {
-        stg<true>(&a[o + off], ta[g]);
+        stg<true>(&a[o + off], x);
       }
     }
+  }
+  if (norm_out) {
+    for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int i = 0; i < kSwapThreads / 64; ++i) t += red[i];
+      norm_out[blockIdx.x] = t;
+    }
+  }
+}
+
+// sum of the partials (in order) and the renormalisation factor: out[0] =
+// norm^2, out[1] = 1 / norm (1 for a zero state, which stays unchanged)
+static __global__ __launch_bounds__(64) void k_norm_scale(const double* __restrict__ partials, int count,
+                                                         double* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < count; ++i) t += partials[i];
+    out[0] = t;
+    out[1] = t > 0.0 ? 1.0 / sqrt(t) : 1.0;
   }
 }
 

@@ -66,8 +66,12 @@ typedef struct hsv_gate {
 enum hsv_fused_mode {
   HSV_FUSED_F64 = 0,        /* FP64 matrix cores (v_mfma_f64)            */
   HSV_FUSED_F32SPLIT = 1,   /* FP32 matrix cores, hi/lo split products   */
-  HSV_FUSED_BF16SPLIT = 2   /* BF16 matrix cores, hi/lo split products
+  HSV_FUSED_BF16SPLIT = 2,  /* BF16 matrix cores, hi/lo split products
                                accumulated in FP32; parts of 5..14 qubits */
+  /* hsv_apply_fused_chain only, OR-ed into the mode: end with the state
+   * renormalised (as hsv_renormalize, but folded into the layout-restoring
+   * swap passes when there are two; a zero state stays as it is). */
+  HSV_CHAIN_RENORMALIZE = 0x100
 };
 
 /* Replaces StateVector(Qubit, int alloc) (state_vector.hpp:60-61, 74-119,

@@ -209,6 +209,38 @@ def test_fused_chain_matches_oracle(orc, n, depth, k, dtype):
     assert np.abs(got - want).max() <= 1e-6
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("n,depth,k", [(14, 8, 12), (20, 12, 13), (22, 10, 13), (21, 9, 12)])
+def test_fused_chain_renormalize(orc, n, depth, k, dtype):
+    """HSV_CHAIN_RENORMALIZE: the chain ends on a unit-norm state (folded
+    into the swap passes of the layout restore, or a separate renormalisation
+    where the restore is one tile pass); equal to the chain followed by
+    hsv_renormalize, and within 1e-6 of the FP64 oracle."""
+    from hisvsim_amd.partition import chain_parts
+    gates = random_layered(n, depth, seed=7 * n + k)
+    parts, lows = chain_parts(gates, k)
+    a = rand_state(n, k)
+    want = a.copy()
+    orc.apply_gates(want, gates)
+    got = gpu_run(n, a, lambda s: s.apply_fused_chain(parts, lows, renormalize=True), dtype)
+    two = gpu_run(n, a, lambda s: [s.apply_fused_chain(parts, lows), s.renormalize()], dtype)
+    eps = 1e-12 if dtype == "complex128" else 1e-5
+    assert abs(np.vdot(got, got).real - 1) <= eps
+    assert np.abs(got - two).max() <= (1e-14 if dtype == "complex128" else 1e-6)
+    assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
+    assert np.abs(got - want).max() <= 1e-6
+
+
+def test_fused_chain_renormalize_zero_state():
+    """A zero state stays zero (no error) under the renormalising chain."""
+    from hisvsim_amd.partition import chain_parts
+    n = 20
+    gates = random_layered(n, 8, seed=5)
+    parts, lows = chain_parts(gates, 13)
+    got = gpu_run(n, np.zeros(2 ** n, complex), lambda s: s.apply_fused_chain(parts, lows, renormalize=True))
+    assert not got.any()
+
+
 @pytest.mark.parametrize("prep", ["reset", "reset+x", "write"])
 def test_fused_chain_from_basis_state(orc, prep):
     """From |0...0> (after reset) the chain's first parts skip the tiles that
