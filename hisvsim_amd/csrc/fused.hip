@@ -459,18 +459,21 @@ struct PartBfTables {
 };
 // 512 threads: 8 amplitudes per thread for 12-qubit tiles, 16 for 13; 1024
 // threads (16 amplitudes each) for 14. 12-qubit tiles keep every step's U
-// fragments in LDS; 13-qubit tiles (64 KB of FP32 in LDS) stream them
-// through one 8 KB buffer, each of 512 threads holding its 16-byte share of
-// the next step in a register, so two workgroups fit a CU and one computes
-// while the other loads or stores. A 14-qubit tile (128 KB) fills the CU
-// alone, so its compute does not overlap its I/O (66 ms against 54 ms per
-// pass at 33 qubits): it saves passes only where parts get much wider.
-constexpr int part_bf_threads(int kt) { return kt >= 14 ? 1024 : 512; }
+// fragments in LDS; 13-qubit tiles (64 KB of FP32 in LDS) either stream them
+// (STREAM_U) through one 8 KB buffer, each of 512 threads holding its
+// 16-byte share of the next step in a register, so two workgroups fit a CU
+// and one computes while the other loads or stores; or keep them resident
+// with 1024 threads (8 amplitudes each) and one workgroup per CU, which
+// then loads the next tile into registers during the steps, as 12-qubit
+// tiles do. A 14-qubit tile (128 KB) fills the CU alone and streams, so its
+// compute does not overlap its I/O (66 ms against 54 ms per pass at 33
+// qubits): it saves passes only where parts get much wider.
+constexpr int part_bf_threads(int kt, bool stream_u) { return kt >= 14 || (kt == 13 && !stream_u) ? 1024 : 512; }
 constexpr int kPartBfMaxSteps = 12;
 
-template <typename R, int KT>
-// two workgroups of 8 waves per CU: at most 128 VGPRs
-__global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_per_eu(4))) void k_part_bf(typename VecT<R>::type* __restrict__ a,
+template <typename R, int KT, bool STREAM_U>
+// 16 waves per CU (two workgroups of 8, or one of 16): at most 128 VGPRs
+__global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgpu_waves_per_eu(4))) void k_part_bf(typename VecT<R>::type* __restrict__ a,
                                                           const PartBfTables* __restrict__ tabs,
                                                           const PartBfStep* __restrict__ steps_g,
                                                           const bf8* __restrict__ ufrag, int nsteps,
@@ -478,11 +481,11 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
                                                           int tiles_per_wg) {
   constexpr int T = 1 << KT;
   constexpr int NCB = (1 << (KT - 5)) / 16;   // column blocks of a step
-  constexpr int NTH = part_bf_threads(KT);
+  constexpr int NTH = part_bf_threads(KT, STREAM_U);
   constexpr int W = NTH / 64;
   constexpr int CBW = NCB / W;                // column blocks per wave
   constexpr int PER = T / NTH;
-  constexpr bool STREAM_U = KT >= 13;
+  static_assert(KT == 13 || STREAM_U == (KT >= 14), "12-qubit tiles keep U resident, 14-qubit tiles stream it");
   constexpr int STEP_FRAGS = 2 * 4 * 64;      // bf8 per step: [rb][set][lane]
   static_assert(CBW >= 1 && NCB % W == 0 && PER * NTH == T, "tile shape");
   static_assert(!STREAM_U || STEP_FRAGS <= NTH, "at most one fragment per thread per step");
@@ -523,37 +526,33 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
   if constexpr (STREAM_U)
     if (threadIdx.x < STEP_FRAGS) fr = ufrag[threadIdx.x];
   __syncthreads();  // tables ready
+  // Thread x handles tile positions t = x + NTH j: t & 63 is its lane and
+  // t >> 6 = wave + W j, so the per-j table entries sit at fixed offsets
+  // from one base (no per-j address registers).
+  const uint64_t a_lo = alo[lane];
+  const int p_lo = plo[lane];
   V raw[PER];
   if constexpr (!STREAM_U) {
     const uint64_t base = tile_base(t0, outer_mask, top);
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      const int t = threadIdx.x + NTH * j;
-      raw[j] = ld_stream<R>(&a[base + alo[t & 63] + ahi[t >> 6]]);
-    }
+    for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a[base + a_lo + ahi[wave + W * j]]);
   }
   for (int it = 0; it < nt; ++it) {
     const uint64_t base = tile_base(t0 + it, outer_mask, top);
     if constexpr (STREAM_U) {
 #pragma unroll
-      for (int j = 0; j < PER; ++j) {
-        const int t = threadIdx.x + NTH * j;
-        raw[j] = ld_stream<R>(&a[base + alo[t & 63] + ahi[t >> 6]]);
-      }
+      for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a[base + a_lo + ahi[wave + W * j]]);
     }
     if (it > 0) __syncthreads();  // the previous tile's stores have read ys
+    // t ^ swz[t >> 4] with swz < 32 and NTH j a multiple of 512
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      const int t = threadIdx.x + NTH * j;
-      ys[t ^ swz[t >> 4]] = make_float2((float)raw[j].x, (float)raw[j].y);
-    }
+    for (int j = 0; j < PER; ++j)
+      ys[(threadIdx.x ^ swz[(threadIdx.x >> 4) + (NTH >> 4) * j]) + NTH * j] =
+          make_float2((float)raw[j].x, (float)raw[j].y);
     if (!STREAM_U && it + 1 < nt) {
       const uint64_t nbase = tile_base(t0 + it + 1, outer_mask, top);
 #pragma unroll
-      for (int j = 0; j < PER; ++j) {
-        const int t = threadIdx.x + NTH * j;
-        raw[j] = ld_stream<R>(&a[nbase + alo[t & 63] + ahi[t >> 6]]);
-      }
+      for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a[nbase + a_lo + ahi[wave + W * j]]);
     }
     for (int s = 0; s < nsteps; ++s) {
       __syncthreads();  // the tile (or the previous step's results) is in LDS
@@ -625,13 +624,12 @@ __global__ __launch_bounds__(part_bf_threads(KT)) __attribute__((amdgpu_waves_pe
     __syncthreads();
 #pragma unroll 8
     for (int j = 0; j < PER; ++j) {
-      const int u = threadIdx.x + NTH * j;
-      const int t = plo[u & 63] | phi[u >> 6];
+      const int t = p_lo | phi[wave + W * j];
       const float2 y = ys[t ^ swz[t >> 4]];
       V o;
       o.x = (R)y.x;
       o.y = (R)y.y;
-      st_stream<R>(&a[base + alo[u & 63] + ahi[u >> 6]], o);
+      st_stream<R>(&a[base + a_lo + ahi[wave + W * j]], o);
     }
   }
 }
@@ -898,18 +896,18 @@ void choose_part_swizzle(int KT, const std::vector<PartSwizzleKey>& keys, uint32
   }
 }
 
-template <typename R, int KT>
+template <typename R, int KT, bool STREAM_U>
 void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t steps_off, size_t frag_off,
                     int nsteps, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer, uint64_t top, int tpw,
                     int* rc) {
-  const size_t lds = KT >= 13 ? 0 : size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);  // 13, 14: fragments streamed
-  auto kern = &k_part_bf<R, KT>;
+  const size_t lds = STREAM_U ? 0 : size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);
+  auto kern = &k_part_bf<R, KT, STREAM_U>;
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)std::max<size_t>(lds, 1)) != hipSuccess) {
     *rc = HSV_ERR_DEVICE;
     return;
   }
-  hipLaunchKernelGGL(kern, grid, part_bf_threads(KT), lds, st, static_cast<typename VecT<R>::type*>(amp),
+  hipLaunchKernelGGL(kern, grid, part_bf_threads(KT, STREAM_U), lds, st, static_cast<typename VecT<R>::type*>(amp),
                      reinterpret_cast<const PartBfTables*>(dev + tabs_off),
                      reinterpret_cast<const PartBfStep*>(dev + steps_off),
                      reinterpret_cast<const bf8*>(dev + frag_off), nsteps, ntiles, outer, top, tpw);
@@ -1091,7 +1089,10 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   const size_t static_lds =
       (size_t(1) << KT) * 8 + sizeof(PartBfStep) * kPartBfMaxSteps + (64 + (size_t(1) << (KT - 6))) * 10 +
       (size_t(1) << (KT - 4));
-  const int max_steps = KT >= 13 ? kPartBfMaxSteps
+  // 13-qubit tiles: resident fragments (1024 threads) when the part's steps
+  // fit LDS beside the tile, else streamed
+  const bool stream_u = KT >= 14 || (KT == 13 && (bf.part13 == 0 || (160 * 1024 - static_lds) / (2 * 4 * 64 * 16) < size_t(ns)));
+  const int max_steps = stream_u ? kPartBfMaxSteps
                                  : (int)std::min<size_t>(kPartBfMaxSteps, (160 * 1024 - static_lds) / (2 * 4 * 64 * 16));
   const size_t steps_off = 2 * sizeof(PartBfTables);
   const size_t frag_off = (steps_off + sizeof(PartBfStep) * ns + 255) & ~size_t(255);
@@ -1111,7 +1112,7 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   // stay zero): they are skipped
   const uint64_t outer = all & ~tmask & live_slots;
   const uint64_t ntiles = 1ull << __builtin_popcountll(outer);
-  if (tpw <= 0) tpw = KT == 12 ? 8 : 1;
+  if (tpw <= 0) tpw = stream_u ? 1 : 8;
   const int grid = (int)((ntiles + tpw - 1) / tpw);
   const uint64_t top = top_slots(outer, bf.tile_order);
   const auto* dev = static_cast<const unsigned char*>(scratch->ptr);
@@ -1124,12 +1125,16 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     const size_t to = s0 + cnt >= ns ? sizeof(PartBfTables) : 0;
     const size_t so = steps_off + sizeof(PartBfStep) * s0;
     const size_t fo = frag_off + size_t(s0) * 2 * 4 * 64 * 16;
-    if (KT == 12) c128 ? launch_part_bf<double, 12>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
-                       : launch_part_bf<float, 12>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
-    else if (KT == 13) c128 ? launch_part_bf<double, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
-                            : launch_part_bf<float, 13>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
-    else c128 ? launch_part_bf<double, 14>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
-              : launch_part_bf<float, 14>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
+    if (KT == 12) c128 ? launch_part_bf<double, 12, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+                       : launch_part_bf<float, 12, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
+    else if (KT == 13 && stream_u)
+      c128 ? launch_part_bf<double, 13, true>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+           : launch_part_bf<float, 13, true>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
+    else if (KT == 13)
+      c128 ? launch_part_bf<double, 13, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+           : launch_part_bf<float, 13, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
+    else c128 ? launch_part_bf<double, 14, true>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+              : launch_part_bf<float, 14, true>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
   }
   if (rc) { *err = "wide bf16 fused kernel attribute setup failed"; return rc; }
   const hipError_t e = hipGetLastError();

@@ -26,8 +26,11 @@ struct BfTune {
   int tile_order = 1;  // tile kernels: 1 = the highest slot outside the tile
                        // varies fastest across workgroups, 0 = natural order
   int part_tpw = 0;  // tiles per workgroup, wide-part kernel (8..14 qubits);
-                     // 0: 8 for 12-qubit tiles (the next tile is prefetched
-                     // during the steps), 1 for 13- and 14-qubit tiles
+                     // 0: 8 where the step fragments stay in LDS (the next
+                     // tile is prefetched during the steps), else 1
+  int part13 = 0;    // 13-qubit tiles: 0 = 512 threads, fragments streamed,
+                     // two workgroups per CU; 1 = 1024 threads, fragments
+                     // resident, next tile prefetched (when the steps fit)
 };
 
 // Applies the fused unitary of `gates` to the n-qubit state at `amp`

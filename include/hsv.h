@@ -215,9 +215,14 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *   "fused_tiles_per_wg" 4096-amplitude tiles each workgroup of the tile
  *                 form takes in turn (1..64, default 8);
  *   "part_tiles_per_wg"  the same for the wide BF16-split part kernel
- *                 (parts of 8..14 qubits; 0..64; 0, the default, = 8 for
- *                 12-qubit tiles, whose next tile is prefetched during the
- *                 steps, and 1 for 13- and 14-qubit tiles);
+ *                 (parts of 8..14 qubits; 0..64; 0, the default, = 8
+ *                 where the next tile is prefetched during the steps, 1
+ *                 where the U fragments are streamed);
+ *   "part13_resident" 13-qubit tiles of the wide part kernel: 1 = 1024
+ *                 threads, every step's U fragments in LDS, the next tile
+ *                 prefetched into registers during the steps (parts of up
+ *                 to 11 steps); 0 = 512 threads, U streamed, two
+ *                 workgroups per CU;
  *   "tile_order"  order in which the BF16 tile kernels' workgroups take
  *                 tiles: 1..3 = the highest 1..3 slots outside the tile vary
  *                 fastest, so concurrent tiles spread over the halves

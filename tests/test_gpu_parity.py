@@ -231,6 +231,32 @@ def test_fused_chain_renormalize(orc, n, depth, k, dtype):
     assert np.abs(got - want).max() <= 1e-6
 
 
+@pytest.mark.parametrize("tpw", [0, 3])
+@pytest.mark.parametrize("n,depth,k", [(16, 12, 13), (20, 12, 13), (19, 30, 13)])
+def test_fused_chain_part13_resident(orc, n, depth, k, tpw):
+    """The 13-qubit wide part kernel with its U fragments resident in LDS
+    (1024 threads, next tile prefetched during the steps; part13_resident=1),
+    with the default and a ragged number of tiles per workgroup: equal to the
+    streamed form within FP32 rounding and within 1e-6 of the FP64 oracle."""
+    from hisvsim_amd.partition import chain_parts
+    gates = random_layered(n, depth, seed=11 * n + depth)
+    parts, lows = chain_parts(gates, k)
+    a = rand_state(n, depth)
+    want = a.copy()
+    orc.apply_gates(want, gates)
+
+    def run(resident):
+        def fn(s):
+            s.tune("part13_resident", resident)
+            s.tune("part_tiles_per_wg", tpw)
+            s.apply_fused_chain(parts, lows, renormalize=True)
+        return gpu_run(n, a, fn)
+    got, streamed = run(1), run(0)
+    assert np.abs(got - streamed).max() <= 1e-6
+    assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
+    assert np.abs(got - want).max() <= 1e-6
+
+
 def test_fused_chain_renormalize_zero_state():
     """A zero state stays zero (no error) under the renormalising chain."""
     from hisvsim_amd.partition import chain_parts
