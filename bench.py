@@ -217,7 +217,8 @@ def main() -> int:
     ap.add_argument("--k", type=int, default=0, help="part width (default: per mode; permute: qubits exchanged)")
     ap.add_argument("--partitioner", default="best", choices=["best", "greedy", "dfs", "closure"],
                     help="acyclic partitioner (best: the one with fewer parts)")
-    ap.add_argument("--pergate-layers", type=int, default=1, help="layers of the per-gate timing sample")
+    ap.add_argument("--pergate-layers", type=int, default=1,
+                    help="layers of the per-gate timing sample (0: no sample)")
     ap.add_argument("--layout", default="chain", choices=["chain", "fixed"],
                     help="wide BF16-split parts (k >= 8): moving low-slot layout (chain) or qubits 0-4 "
                          "in every part (fixed)")
@@ -304,7 +305,7 @@ def main() -> int:
                 f"max|d| {full_verify['max_abs_err']:.2e}")
 
         per_gate = None
-        if args.mode != "permute":
+        if args.mode != "permute" and args.pergate_layers > 0:
             # per-gate sample: first layers of the same circuit
             per_layer = n + n // 2
             sample = gates[: per_layer * args.pergate_layers]

@@ -444,16 +444,19 @@ struct SwapTiles {
   uint8_t pa[32], pb[32];
 };
 constexpr int kSwapThreads = 256;
+constexpr int kSwapPer = 1024 / kSwapThreads;  // positions per thread and tile (nb <= 10)
 // Optionally fused with a renormalisation: norm_out (one FP64 partial of
 // sum |a|^2 per workgroup, in a fixed order) in one pass, scale (a device
-// value, multiplied into every stored amplitude) in the next.
+// value, multiplied into every stored amplitude) in the next. The next pair
+// of tiles is loaded into registers while the current one is stored, so
+// every workgroup keeps loads in flight through its store phase.
 template <typename R>
 __global__ __launch_bounds__(kSwapThreads) void k_swap_tiles(typename VecT<R>::type* __restrict__ a, uint64_t nouter,
                                                            SwapTiles st, double* __restrict__ norm_out,
                                                            const double* __restrict__ scale) {
   using V = typename VecT<R>::type;
+  // 40 KB of LDS in all (FP64): four workgroups per CU
   __shared__ V ta[1024], tb[1024];
-  __shared__ double red[kSwapThreads / 64];
   double acc = 0.0;
   const R sc = scale ? R(*scale) : R(1);
   // outer address of tile w and of its partner, per 7-bit chunk of w
@@ -470,43 +473,75 @@ __global__ __launch_bounds__(kSwapThreads) void k_swap_tiles(typename VecT<R>::t
     ot2[c][v] = o2;
   }
   const int T = 1 << st.nb;
-  for (uint64_t w = blockIdx.x; w < nouter; w += gridDim.x) {
-    const int w0 = w & 127, w1 = (w >> 7) & 127, w2 = (w >> 14) & 127, w3 = (w >> 21) & 127;
-    __syncthreads();  // tables built; the previous tile's LDS reads are done
-    const uint64_t o = ot[0][w0] | ot[1][w1] | ot[2][w2] | ot[3][w3];
-    const uint64_t o2 = ot2[0][w0] | ot2[1][w1] | ot2[2][w2] | ot2[3][w3];
-    if (o2 < o) continue;  // the partner tile's workgroup does this pair
-    const bool pair = o2 != o;
-    for (int f = threadIdx.x; f < T; f += kSwapThreads) {
-      const uint64_t off = st.lo[f & 31] + st.hi[f >> 5];
-      const V x = ldg<true>(&a[o + off]);
-      ta[f] = x;
-      if (norm_out) acc += double(x.x) * double(x.x) + double(x.y) * double(x.y);
-      if (pair) {
-        const V y = ldg<true>(&a[o2 + off]);
-        tb[f] = y;
-        if (norm_out) acc += double(y.x) * double(y.x) + double(y.y) * double(y.y);
+  __syncthreads();  // tables built
+  // the workgroup's tiles w = blockIdx.x + j gridDim.x whose partner is not
+  // below them (the partner tile's workgroup does the others); uniform
+  uint64_t w = blockIdx.x, o = 0, o2 = 0;
+  auto advance = [&]() {
+    for (; w < nouter; w += gridDim.x) {
+      const int w0 = w & 127, w1 = (w >> 7) & 127, w2 = (w >> 14) & 127, w3 = (w >> 21) & 127;
+      o = ot[0][w0] | ot[1][w1] | ot[2][w2] | ot[3][w3];
+      o2 = ot2[0][w0] | ot2[1][w1] | ot2[2][w2] | ot2[3][w3];
+      if (o2 >= o) return;
+    }
+  };
+  V xa[kSwapPer], xb[kSwapPer];
+  auto load = [&]() {
+#pragma unroll
+    for (int i = 0; i < kSwapPer; ++i) {
+      const int f = threadIdx.x + kSwapThreads * i;
+      if (f < T) {
+        const uint64_t off = st.lo[f & 31] + st.hi[f >> 5];
+        xa[i] = ldg<true>(&a[o + off]);
+        if (o2 != o) xb[i] = ldg<true>(&a[o2 + off]);
       }
     }
-    __syncthreads();
-    for (int f = threadIdx.x; f < T; f += kSwapThreads) {
-      const uint64_t off = st.lo[f & 31] + st.hi[f >> 5];
-      const int g = st.slo[f & 31] | st.shi[f >> 5];
-      V x = ta[g];
-      x.x *= sc;
-      x.y *= sc;
-      if (pair) {
-        V y = tb[g];
-        y.x *= sc;
-        y.y *= sc;
-        stg<true>(&a[o2 + off], x);
-        stg<true>(&a[o + off], y);
-      } else {
-        stg<true>(&a[o + off], x);
+  };
+  advance();
+  if (w < nouter) load();
+  while (w < nouter) {
+    const uint64_t co = o, co2 = o2;
+    const bool pair = co2 != co;
+#pragma unroll
+    for (int i = 0; i < kSwapPer; ++i) {
+      const int f = threadIdx.x + kSwapThreads * i;
+      if (f < T) {
+        ta[f] = xa[i];
+        if (norm_out) acc += double(xa[i].x) * double(xa[i].x) + double(xa[i].y) * double(xa[i].y);
+        if (pair) {
+          tb[f] = xb[i];
+          if (norm_out) acc += double(xb[i].x) * double(xb[i].x) + double(xb[i].y) * double(xb[i].y);
+        }
       }
     }
+    __syncthreads();  // the tiles are in LDS
+    w += gridDim.x;
+    advance();
+    if (w < nouter) load();  // in flight during the stores below
+#pragma unroll
+    for (int i = 0; i < kSwapPer; ++i) {
+      const int f = threadIdx.x + kSwapThreads * i;
+      if (f < T) {
+        const uint64_t off = st.lo[f & 31] + st.hi[f >> 5];
+        const int g = st.slo[f & 31] | st.shi[f >> 5];
+        V x = ta[g];
+        x.x *= sc;
+        x.y *= sc;
+        if (pair) {
+          V y = tb[g];
+          y.x *= sc;
+          y.y *= sc;
+          stg<true>(&a[co2 + off], x);
+          stg<true>(&a[co + off], y);
+        } else {
+          stg<true>(&a[co + off], x);
+        }
+      }
+    }
+    __syncthreads();  // LDS reads done before the next tile's writes
   }
   if (norm_out) {
+    double* red = reinterpret_cast<double*>(ta);  // free: the loop ended on a barrier
     for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();

@@ -94,8 +94,9 @@ def traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, kernel, cal_kern
             "fetch_bytes": fk * ff, "write_bytes": wk * wf, "bytes_per_launch": fk * ff + wk * wf}
 
 
-def trace_durations(d: str, kernel: str, min_ms: float = 0.0) -> List[float]:
-    """Per-dispatch durations (ms) of a kernel, in start order, from the
+def trace_dispatches(d: str, kernel: str) -> List[tuple]:
+    """(duration ms, grid size, kernel name) per dispatch of the kernels
+    whose name contains KERNEL, in start order, from the
     ``*_kernel_trace.csv`` under DIR."""
     rows = []
     for path in _csvs(d, "kernel_trace.csv"):
@@ -103,20 +104,42 @@ def trace_durations(d: str, kernel: str, min_ms: float = 0.0) -> List[float]:
             for r in csv.DictReader(f):
                 if kernel in r["Kernel_Name"]:
                     ms = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
-                    if ms >= min_ms:
-                        rows.append((int(r["Start_Timestamp"]), ms))
-    return [ms for _, ms in sorted(rows)]
+                    grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+                    rows.append((int(r["Start_Timestamp"]), ms, grid, short_name(r["Kernel_Name"])))
+    return [row[1:] for row in sorted(rows)]
+
+
+def trace_durations(d: str, kernel: str, min_ms: float = 0.0) -> List[float]:
+    """Per-dispatch durations (ms) of a kernel, in start order."""
+    return [row[0] for row in trace_dispatches(d, kernel) if row[0] >= min_ms]
 
 
 def steps_table(d: str, n: int, depth: int, k: int) -> dict:
+    """Pass time of the first circuit's part passes by the number of 5-qubit
+    steps in the part (the trace of one bench run of the chain). Passes over
+    only part of the state (the first passes from |0...0>, which skip the
+    tiles that are still zero: a smaller grid than the full passes of the
+    kernel instantiation) are left out."""
     from .circuit import random_layered
     from .partition import chain_parts, closure_parts
     parts, _ = chain_parts(random_layered(n, depth, seed=0), k)
-    durs = trace_durations(d, "k_part_bf", min_ms=1.0)[:len(parts)]
+    disp = trace_dispatches(d, "k_part_bf")[:len(parts)]
+    max_grid: Dict[str, int] = {}
+    for _, grid, name in disp:
+        max_grid[name] = max(max_grid.get(name, 0), grid)
     by: Dict[int, List[float]] = {}
-    for p, ms in zip(parts, durs):
+    partial = 0
+    for p, (ms, grid, name) in zip(parts, disp):
+        if grid < max_grid[name]:
+            partial += 1
+            continue
         by.setdefault(len(closure_parts(p, 5)), []).append(ms)
-    return {"parts": len(parts), "dispatches": len(durs),
+    every = trace_dispatches(d, "k_part_bf")
+    for _, grid, name in every:
+        max_grid[name] = max(max_grid.get(name, 0), grid)
+    full = [ms for ms, grid, name in every if grid == max_grid[name]]
+    return {"parts": len(parts), "dispatches": len(disp), "partial_passes_left_out": partial,
+            "full_passes_in_trace": len(full), "full_pass_mean_ms": statistics.mean(full) if full else None,
             "by_steps": {s: {"count": len(v), "mean_ms": statistics.mean(v), "min_ms": min(v), "max_ms": max(v)}
                          for s, v in sorted(by.items())}}
 
