@@ -50,15 +50,16 @@ def log(msg: str) -> None:
 LOW_SLOTS = 5  # qubits the device adds to every LDS-staged part (libhsv default)
 
 
-def make_parts(gates, mode: str, k: int, partitioner: str = "best", layout: str = "fixed"):
+def make_parts(gates, mode: str, k: int, partitioner: str = "best", layout: str = "fixed", beam=(8, 4)):
     """Acyclic parts: DAG-aware greedy, depth-first order, qubit-set
     closure, or whichever gives the fewest parts ("best"). For the register
     part kernel, k counts the low qubits the device adds to every part
     anyway, so the partitioner starts each part with them. With the moving
     layout (wide BF16-split parts only) the parts come from chain_parts and
-    the low slots follow them. Returns (parts, partitioner, lows or None)."""
+    the low slots follow them (beam: width and branching of the planner's
+    search over low sets). Returns (parts, partitioner, lows or None)."""
     if mode == "fusedbf16" and k >= 8 and layout == "chain":
-        parts, lows = chain_parts(gates, k, LOW_SLOTS)
+        parts, lows = chain_parts(gates, k, LOW_SLOTS, beam=beam)
         return parts, "chain", lows
     # the register kernel and the wide (k >= 8) BF16-split kernel load whole
     # parts as tiles, so their parts start with the low qubits for long runs
@@ -219,6 +220,8 @@ def main() -> int:
                     help="acyclic partitioner (best: the one with fewer parts)")
     ap.add_argument("--pergate-layers", type=int, default=1,
                     help="layers of the per-gate timing sample (0: no sample)")
+    ap.add_argument("--beam", default="8,4", metavar="WIDTH,BRANCH",
+                    help="chain planner: beam search over the low sets (1,1 = greedy)")
     ap.add_argument("--layout", default="chain", choices=["chain", "fixed"],
                     help="wide BF16-split parts (k >= 8): moving low-slot layout (chain) or qubits 0-4 "
                          "in every part (fixed)")
@@ -242,7 +245,8 @@ def main() -> int:
     elif args.mode == "pergate":
         parts = []
     else:
-        parts, partitioner_used, lows = make_parts(gates, args.mode, k, args.partitioner, args.layout)
+        beam = tuple(int(x) for x in args.beam.split(","))
+        parts, partitioner_used, lows = make_parts(gates, args.mode, k, args.partitioner, args.layout, beam)
     S = 16
     total_upd = sum(gate_amplitudes(g, n) for g in gates)
     log(f"[bench] n={n} depth={depth} gates={len(gates)} mode={args.mode} k={k} parts={len(parts)}")
@@ -382,7 +386,7 @@ def main() -> int:
         "data": "synthetic: seeded random layered circuit (U3 angles ~ U[0,2pi), CZ on random matchings), |0..0> input",
         "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, complex128 state "
                                 f"({2 ** n * 16 / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
-                    "mode": args.mode, "k": k, "passes": passes, "partitioner": partitioner_used,
+                    "mode": args.mode, "k": k, "passes": passes, "partitioner": partitioner_used if lows is None else f"chain (beam {beam[0]}x{beam[1]})",
                     **({"layout": "chain (moving low slots, one restoring permutation pass)",
                         "layout_restore_ms": restore_ms, "partial_passes_from_zero": partial}
                        if lows is not None else {}),

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from typing import Iterable, Sequence
+from typing import Iterable, Sequence, Tuple
 
 import numpy as np
 
@@ -91,8 +91,8 @@ SIGNATURES = {
     "hsv_apply_fused": (ctypes.c_int, [_P, _G, ctypes.c_int, ctypes.c_int]),
     "hsv_apply_fused_chain": (ctypes.c_int, [_P, _G, ctypes.c_int, _I, ctypes.c_int, _I, ctypes.c_int]),
     "hsv_chain_parts": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                       ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
-                                       ctypes.POINTER(ctypes.c_int32), _I]),
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _I]),
     "hsv_permute_qubits": (ctypes.c_int, [_P, _I]),
     "hsv_read": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
     "hsv_write": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
@@ -192,14 +192,17 @@ def gate_matrix(g) -> np.ndarray:
     return m[:k * k].reshape(k, k)
 
 
-def chain_parts(gates: Sequence, k: int, low: int = 5, lookahead: int = 7, candidates: int = 24):
-    """hsv_chain_parts (host only): (parts, lows) as partition.chain_parts."""
+def chain_parts(gates: Sequence, k: int, low: int = 5, lookahead: int = 7, candidates: int = 24,
+                beam: Tuple[int, int] = (1, 1)):
+    """hsv_chain_parts (host only): (parts, lows) as partition.chain_parts;
+    beam = (width, branch) of the search over low sets, (1, 1) = greedy."""
     masks = np.array([sum(1 << q for q in set(g.qubits())) for g in gates], dtype=np.uint64)
     part = np.zeros(max(1, len(gates)), np.int32)
     lows = np.zeros(max(1, len(gates)) * low + low, np.int32)
     num = ctypes.c_int(0)
     _check(lib().hsv_chain_parts(masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(gates), k, low,
-                                 lookahead, candidates, part.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                 lookahead, candidates, int(beam[0]), int(beam[1]),
+                                 part.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                  lows.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(num)))
     parts = [[] for _ in range(num.value)]
     for g, pi in zip(gates, part[:len(gates)]):

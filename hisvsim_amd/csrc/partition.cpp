@@ -62,81 +62,153 @@ uint64_t grow(const std::vector<uint64_t>& qm, const std::vector<int>& rem, uint
   return Q;
 }
 
+// One part of the chain from state (rem, lowset): the part's gates (taken),
+// the gates left (keep) and the part's tile.
+void take_part(const std::vector<uint64_t>& qm, const std::vector<int>& rem, uint64_t lowset, int k, int candidates,
+               std::vector<int>* taken, std::vector<int>* keep, uint64_t* tile) {
+  taken->clear();
+  absorb(qm, rem, grow(qm, rem, lowset, k, candidates), taken);
+  *tile = lowset;
+  for (int i : *taken) *tile |= qm[i];
+  keep->clear();
+  keep->reserve(rem.size());
+  size_t ti = 0;
+  for (int i : rem) {
+    if (ti < taken->size() && (*taken)[ti] == i) {
+      ++ti;
+      continue;
+    }
+    keep->push_back(i);
+  }
+}
+
+// The candidate low sets for the part after one with this tile: every
+// `low`-subset of the first `lookahead` tile qubits ordered by the position
+// of their next gate (then index), in lexicographic order, each with the
+// gates the next part would absorb; sorted by that count, descending, ties
+// in enumeration order.
+std::vector<std::pair<int, uint64_t>> rank_lows(const std::vector<uint64_t>& qm, const std::vector<int>& rem,
+                                                uint64_t tile, int k, int low, int lookahead, int candidates) {
+  std::vector<int> first(64, (int)rem.size());
+  for (int pos = (int)rem.size() - 1; pos >= 0; --pos) {
+    uint64_t g = qm[rem[pos]] & tile;
+    while (g) {
+      first[__builtin_ctzll(g)] = pos;
+      g &= g - 1;
+    }
+  }
+  std::vector<int> order;
+  for (int b = 0; b < 64; ++b)
+    if (tile >> b & 1) order.push_back(b);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return first[a] < first[b]; });
+  const int m = std::min<int>(lookahead, (int)order.size());
+  std::vector<std::pair<int, uint64_t>> out;
+  std::vector<int> idx(low);
+  for (int i = 0; i < low; ++i) idx[i] = i;
+  while (true) {
+    uint64_t cand = 0;
+    for (int i : idx) cand |= 1ull << order[i];
+    out.push_back({absorb(qm, rem, grow(qm, rem, cand, k, candidates), nullptr), cand});
+    int i = low - 1;
+    while (i >= 0 && idx[i] == m - low + i) --i;
+    if (i < 0) break;
+    ++idx[i];
+    for (int j = i + 1; j < low; ++j) idx[j] = idx[j - 1] + 1;
+  }
+  std::stable_sort(out.begin(), out.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  return out;
+}
+
+struct ChainState {
+  std::vector<int> rem;
+  uint64_t lowset;
+  std::vector<int32_t> part_of_gate;
+  std::vector<uint64_t> lows;
+};
+
+// Beam search over the low-set choices: every state of the beam takes its
+// next part, each branches into its `branch` best-ranked low sets, and the
+// `width` children with the fewest gates left survive (duplicates dropped).
+// The first state to run out of gates ends the search. Width 1, branch 1 is
+// the greedy chain.
+int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, int candidates, int width, int branch,
+               ChainState* best) {
+  const int num_gates = (int)qm.size();
+  ChainState s0;
+  s0.rem.resize(num_gates);
+  for (int i = 0; i < num_gates; ++i) s0.rem[i] = i;
+  s0.lowset = (1ull << low) - 1;
+  s0.part_of_gate.assign(num_gates, 0);
+  std::vector<ChainState> beam{s0};
+  std::vector<int> taken, keep;
+  while (true) {
+    std::vector<std::pair<int, ChainState>> kids;  // (gates left after the child's next part, child)
+    for (const ChainState& s : beam) {
+      uint64_t tile = 0;
+      take_part(qm, s.rem, s.lowset, k, candidates, &taken, &keep, &tile);
+      if (taken.empty()) continue;  // a gate wider than the part once its lows are in
+      const int p = (int)s.lows.size();
+      if (keep.empty()) {
+        *best = s;
+        for (int i : taken) best->part_of_gate[i] = p;
+        best->lows.push_back(s.lowset);
+        best->rem.clear();
+        return HSV_OK;
+      }
+      const auto ranked = rank_lows(qm, keep, tile, k, low, lookahead, candidates);
+      for (int j = 0; j < branch && j < (int)ranked.size(); ++j) {
+        if (ranked[j].first == 0) break;
+        ChainState c;
+        c.rem = keep;
+        c.lowset = ranked[j].second;
+        c.part_of_gate = s.part_of_gate;
+        for (int i : taken) c.part_of_gate[i] = p;
+        c.lows = s.lows;
+        c.lows.push_back(s.lowset);
+        kids.push_back({(int)keep.size() - ranked[j].first, std::move(c)});
+      }
+    }
+    if (kids.empty()) return HSV_ERR_LIMIT;
+    std::stable_sort(kids.begin(), kids.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    beam.clear();
+    for (auto& kd : kids) {
+      bool dup = false;
+      for (const ChainState& b : beam) dup = dup || (b.lowset == kd.second.lowset && b.rem == kd.second.rem);
+      if (!dup) beam.push_back(std::move(kd.second));
+      if ((int)beam.size() >= width) break;
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" int hsv_chain_parts(const uint64_t* gate_masks, int num_gates, int k, int low, int lookahead,
-                               int candidates, int32_t* part_of_gate, int32_t* lows_out, int* num_parts) {
+                               int candidates, int beam_width, int beam_branch, int32_t* part_of_gate,
+                               int32_t* lows_out, int* num_parts) {
   if (num_gates < 0 || (num_gates && (!gate_masks || !part_of_gate)) || !lows_out || !num_parts || low < 1 ||
-      low > k || k > 63 || lookahead < low || candidates < 1)
+      low > k || k > 63 || lookahead < low || candidates < 1 || beam_width < 1 || beam_branch < 1)
     return HSV_ERR_ARG;
   std::vector<uint64_t> qm(gate_masks, gate_masks + num_gates);
   uint64_t all = 0;
   for (uint64_t g : qm) all |= g;
   if (64 - __builtin_clzll(all | 1) < low) return HSV_ERR_ARG;  // fewer qubits than low slots
-  std::vector<int> rem(num_gates);
-  for (int i = 0; i < num_gates; ++i) rem[i] = i;
-  uint64_t lowset = (1ull << low) - 1;
-  int p = 0;
-  while (!rem.empty()) {
-    std::vector<int> taken;
-    absorb(qm, rem, grow(qm, rem, lowset, k, candidates), &taken);
-    if (taken.empty()) return HSV_ERR_LIMIT;  // a gate wider than the part once its lows are in
-    uint64_t tile = lowset;
-    for (int i : taken) {
-      part_of_gate[i] = p;
-      tile |= qm[i];
-    }
-    for (int b = 0, j = 0; b < 64; ++b)
-      if (lowset >> b & 1) lows_out[p * low + j++] = b;
-    ++p;
-    std::vector<int> keep;
-    keep.reserve(rem.size());
-    size_t ti = 0;
-    for (int i : rem) {
-      if (ti < taken.size() && taken[ti] == i) {
-        ++ti;
-        continue;
-      }
-      keep.push_back(i);
-    }
-    rem.swap(keep);
-    if (rem.empty()) break;
-    // the tile's qubits ordered by the position of their next gate (then index)
-    std::vector<int> first(64, (int)rem.size());
-    for (int pos = (int)rem.size() - 1; pos >= 0; --pos) {
-      uint64_t g = qm[rem[pos]] & tile;
-      while (g) {
-        first[__builtin_ctzll(g)] = pos;
-        g &= g - 1;
-      }
-    }
-    std::vector<int> order;
-    for (int b = 0; b < 64; ++b)
-      if (tile >> b & 1) order.push_back(b);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return first[a] < first[b]; });
-    const int m = std::min<int>(lookahead, (int)order.size());
-    // combinations of `low` of the first m, in lexicographic order; the first
-    // with the most gates absorbed by the next part wins
-    std::vector<int> idx(low);
-    for (int i = 0; i < low; ++i) idx[i] = i;
-    int best_got = -1;
-    uint64_t best = 0;
-    while (true) {
-      uint64_t cand = 0;
-      for (int i : idx) cand |= 1ull << order[i];
-      const int got = absorb(qm, rem, grow(qm, rem, cand, k, candidates), nullptr);
-      if (got > best_got) {
-        best_got = got;
-        best = cand;
-      }
-      int i = low - 1;
-      while (i >= 0 && idx[i] == m - low + i) --i;
-      if (i < 0) break;
-      ++idx[i];
-      for (int j = i + 1; j < low; ++j) idx[j] = idx[j - 1] + 1;
-    }
-    lowset = best;
+  if (num_gates == 0) {
+    *num_parts = 0;
+    return HSV_OK;
   }
-  *num_parts = p;
+  ChainState best;
+  if (int rc = chain_beam(qm, k, low, lookahead, candidates, 1, 1, &best)) return rc;
+  if (beam_width > 1 || beam_branch > 1) {
+    ChainState wide;
+    // the beam can end worse than the greedy chain: keep the shorter
+    if (chain_beam(qm, k, low, lookahead, candidates, beam_width, beam_branch, &wide) == HSV_OK &&
+        wide.lows.size() < best.lows.size())
+      best = std::move(wide);
+  }
+  for (int i = 0; i < num_gates; ++i) part_of_gate[i] = best.part_of_gate[i];
+  for (size_t p = 0; p < best.lows.size(); ++p)
+    for (int b = 0, j = 0; b < 64; ++b)
+      if (best.lows[p] >> b & 1) lows_out[p * low + j++] = b;
+  *num_parts = (int)best.lows.size();
   return HSV_OK;
 }

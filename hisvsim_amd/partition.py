@@ -318,7 +318,8 @@ def _grow(qsets: Sequence[frozenset], remaining: Sequence[int], base: set, k: in
 
 
 def chain_parts(gates: Sequence[Gate], k: int, low: int = 5, lookahead: int = 13,
-                candidates: int = 24, native: bool = True) -> Tuple[List[List[Gate]], List[List[int]]]:
+                candidates: int = 24, native: bool = True,
+                beam: Tuple[int, int] = (1, 1)) -> Tuple[List[List[Gate]], List[List[int]]]:
     """Parts for a moving qubit layout (hsv_apply_fused_chain): part p is
     grown by the ``closure_parts`` method from the ``low`` qubits that sit in
     the lowest slots while it runs (lows[p]; lows[0] = qubits 0..low-1), so
@@ -329,7 +330,11 @@ def chain_parts(gates: Sequence[Gate], k: int, low: int = 5, lookahead: int = 13
     qubits, the low slots then hold qubits the next part uses. Returns
     (parts, lows); the parts are acyclic as for closure_parts. With
     ``native`` the same method runs in libhsv (hsv_chain_parts, ~80x faster,
-    identical result) when the library loads."""
+    identical result) when the library loads. ``beam`` = (width, branch)
+    widens the choice of lows into a beam search (native only: each chain in
+    flight branches into its ``branch`` best low sets, the ``width`` with the
+    fewest gates left go on; the shorter of that and the greedy chain is
+    returned); (1, 1) is the greedy chain above."""
     if native:
         try:
             from . import capi
@@ -339,7 +344,9 @@ def chain_parts(gates: Sequence[Gate], k: int, low: int = 5, lookahead: int = 13
         else:
             if low > k:
                 raise ValueError("more low qubits than the part width")
-            return capi.chain_parts(gates, k, low, max(lookahead, low), candidates)
+            return capi.chain_parts(gates, k, low, max(lookahead, low), candidates, beam)
+    if tuple(beam) != (1, 1):
+        raise ValueError("the beam search over low sets runs in libhsv only")
     from itertools import combinations
     if low > k:
         raise ValueError("more low qubits than the part width")

@@ -187,11 +187,15 @@ int hsv_gate_matrix(const hsv_gate* g, double* out, int* dim);
  * qubit-set closure from the `low` qubits in the low slots (at most k qubits
  * with them); the next part's lows are the `low` of the `lookahead` qubits of
  * the part's tile whose next gates come first that let the next part absorb
- * the most gates. Out: part_of_gate[i] (parts keep the gate order),
- * lows_out[p*low + j] ascending, *num_parts. HSV_ERR_LIMIT: a gate does not
- * fit. Equals hisvsim_amd.partition.chain_parts gate for gate. */
+ * the most gates (beam_width = beam_branch = 1: equals
+ * hisvsim_amd.partition.chain_parts gate for gate). With a wider beam, each
+ * of beam_width chains in flight branches into its beam_branch best low
+ * sets and the beam_width children with the fewest gates left go on; the
+ * result is the shorter of that and the greedy chain. Out: part_of_gate[i]
+ * (parts keep the gate order), lows_out[p*low + j] ascending (room for one
+ * part per gate), *num_parts. HSV_ERR_LIMIT: a gate does not fit. */
 int hsv_chain_parts(const uint64_t* gate_masks, int num_gates, int k, int low, int lookahead, int candidates,
-                    int32_t* part_of_gate, int32_t* lows_out, int* num_parts);
+                    int beam_width, int beam_branch, int32_t* part_of_gate, int32_t* lows_out, int* num_parts);
 int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int* qubits_out,
                       int* num_part_qubits, double* u_out, double* unitarity_err);
 

@@ -219,6 +219,30 @@ def test_chain_parts_native_equals_python(n, depth, k, lookahead):
     assert py[1] == nat[1]
 
 
+@pytest.mark.parametrize("n,depth,k,beam", [(16, 12, 12, (4, 2)), (20, 10, 13, (3, 3)), (14, 8, 10, (2, 4))])
+def test_chain_parts_beam(n, depth, k, beam):
+    """The beam search over low sets (hsv_chain_parts, width and branch > 1):
+    a valid chain (acyclic k-bounded parts, lows[0] = qubits 0-4, each later
+    lows drawn from the previous tile, the gates in circuit order within
+    each part) with no more parts than the greedy chain."""
+    gates = random_layered(n, depth, seed=5 * n + k)
+    greedy, _ = chain_parts(gates, k)
+    parts, lows = chain_parts(gates, k, beam=beam)
+    check_parts(gates, parts, k)
+    assert sum(len(p) for p in parts) == len(gates)
+    assert len(parts) <= len(greedy)
+    assert sorted(lows[0]) == [0, 1, 2, 3, 4]
+    pos = {id(g): i for i, g in enumerate(gates)}
+    for p, (part, low) in enumerate(zip(parts, lows)):
+        assert [pos[id(g)] for g in part] == sorted(pos[id(g)] for g in part)
+        tile = set(low).union(*[set(g.qubits()) for g in part])
+        assert len(set(low)) == 5 and len(tile) <= k
+        if p + 1 < len(parts):
+            assert set(lows[p + 1]) <= tile
+    with pytest.raises(ValueError):
+        chain_parts(gates, k, beam=beam, native=False)
+
+
 def test_chain_parts_apply_like_the_circuit(orc):
     """Applying the chain's parts in order equals the circuit (oracle, n=12)."""
     gates = random_layered(12, 8, seed=4)
