@@ -471,7 +471,7 @@ struct PartBfTables {
 constexpr int part_bf_threads(int kt, bool stream_u) { return kt >= 14 || (kt == 13 && !stream_u) ? 1024 : 512; }
 constexpr int kPartBfMaxSteps = 12;
 
-template <typename R, int KT, bool STREAM_U>
+template <typename R, int KT, bool STREAM_U, bool U_DIRECT = false>
 // 16 waves per CU (two workgroups of 8, or one of 16): at most 128 VGPRs
 __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgpu_waves_per_eu(4))) void k_part_bf(typename VecT<R>::type* __restrict__ a,
                                                           const PartBfTables* __restrict__ tabs,
@@ -489,10 +489,12 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
   constexpr int STEP_FRAGS = 2 * 4 * 64;      // bf8 per step: [rb][set][lane]
   static_assert(CBW >= 1 && NCB % W == 0 && PER * NTH == T, "tile shape");
   static_assert(!STREAM_U || STEP_FRAGS <= NTH, "at most one fragment per thread per step");
+  static_assert(!U_DIRECT || STREAM_U, "direct fragment loads replace the streaming buffer");
+  constexpr bool SU_BUF = STREAM_U && !U_DIRECT;
   using V = typename VecT<R>::type;
   typedef float v4f_t __attribute__((ext_vector_type(4)));
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ bf8 su_one[STREAM_U ? STEP_FRAGS : 1];
+  __shared__ bf8 su_one[SU_BUF ? STEP_FRAGS : 1];
   bf8* su = STREAM_U ? su_one : reinterpret_cast<bf8*>(smem_raw);     // [step][rb][set][lane]
   __shared__ float2 ys[T];
   __shared__ PartBfStep st[kPartBfMaxSteps];
@@ -523,7 +525,7 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
   if (t0 >= ntiles) return;  // uniform across the workgroup
   const int nt = (int)min(uint64_t(tiles_per_wg), ntiles - t0);
   bf8 fr;  // streamed fragments: this thread's share of the next step
-  if constexpr (STREAM_U)
+  if constexpr (SU_BUF)
     if (threadIdx.x < STEP_FRAGS) fr = ufrag[threadIdx.x];
   __syncthreads();  // tables ready
   // Thread x handles tile positions t = x + NTH j: t & 63 is its lane and
@@ -555,10 +557,24 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
       for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a[nbase + a_lo + ahi[wave + W * j]]);
     }
     for (int s = 0; s < nsteps; ++s) {
+      const bf8* As;
+      bf8 Af[2][4];
+      if constexpr (U_DIRECT) {
+        // each wave reads the step's fragments straight from L2 into
+        // registers, issued before the barrier so that their latency
+        // overlaps it
+        As = ufrag + s * STEP_FRAGS + lane;
+        if constexpr (CBW > 1) {
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Af[rb][q] = As[(rb * 4 + q) * 64];
+        }
+      }
       __syncthreads();  // the tile (or the previous step's results) is in LDS
       const PartBfStep& S = st[s];
-      const bf8* As;
-      if constexpr (STREAM_U) {
+      if constexpr (U_DIRECT) {
+      } else if constexpr (STREAM_U) {
         if (threadIdx.x < STEP_FRAGS) {
           su_one[threadIdx.x] = fr;  // every wave is past the previous step's reads
           fr = ufrag[(s + 1 < nsteps ? s + 1 : 0) * STEP_FRAGS + threadIdx.x];
@@ -571,8 +587,7 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
       // with several column blocks per wave, this step's U fragments are
       // read once for all of them (the in-place writes below would otherwise
       // force re-reads)
-      bf8 Af[2][4];
-      if constexpr (CBW > 1) {
+      if constexpr (CBW > 1 && !U_DIRECT) {
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -896,12 +911,12 @@ void choose_part_swizzle(int KT, const std::vector<PartSwizzleKey>& keys, uint32
   }
 }
 
-template <typename R, int KT, bool STREAM_U>
+template <typename R, int KT, bool STREAM_U, bool U_DIRECT = false>
 void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t steps_off, size_t frag_off,
                     int nsteps, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer, uint64_t top, int tpw,
                     int* rc) {
   const size_t lds = STREAM_U ? 0 : size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);
-  auto kern = &k_part_bf<R, KT, STREAM_U>;
+  auto kern = &k_part_bf<R, KT, STREAM_U, U_DIRECT>;
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)std::max<size_t>(lds, 1)) != hipSuccess) {
     *rc = HSV_ERR_DEVICE;
@@ -1091,7 +1106,8 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
       (size_t(1) << (KT - 4));
   // 13-qubit tiles: resident fragments (1024 threads) when the part's steps
   // fit LDS beside the tile, else streamed
-  const bool stream_u = KT >= 14 || (KT == 13 && (bf.part13 == 0 || (160 * 1024 - static_lds) / (2 * 4 * 64 * 16) < size_t(ns)));
+  const bool stream_u = KT >= 14 || (KT == 13 && (bf.part13 != 1 || (160 * 1024 - static_lds) / (2 * 4 * 64 * 16) < size_t(ns)));
+  const bool u_direct = KT == 13 && stream_u && bf.part13 == 2;
   const int max_steps = stream_u ? kPartBfMaxSteps
                                  : (int)std::min<size_t>(kPartBfMaxSteps, (160 * 1024 - static_lds) / (2 * 4 * 64 * 16));
   const size_t steps_off = 2 * sizeof(PartBfTables);
@@ -1127,6 +1143,9 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     const size_t fo = frag_off + size_t(s0) * 2 * 4 * 64 * 16;
     if (KT == 12) c128 ? launch_part_bf<double, 12, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
                        : launch_part_bf<float, 12, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
+    else if (KT == 13 && u_direct)
+      c128 ? launch_part_bf<double, 13, true, true>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+           : launch_part_bf<float, 13, true, true>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
     else if (KT == 13 && stream_u)
       c128 ? launch_part_bf<double, 13, true>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
            : launch_part_bf<float, 13, true>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);

@@ -28,9 +28,11 @@ struct BfTune {
   int part_tpw = 0;  // tiles per workgroup, wide-part kernel (8..14 qubits);
                      // 0: 8 where the step fragments stay in LDS (the next
                      // tile is prefetched during the steps), else 1
-  int part13 = 0;    // 13-qubit tiles: 0 = 512 threads, fragments streamed,
-                     // two workgroups per CU; 1 = 1024 threads, fragments
-                     // resident, next tile prefetched (when the steps fit)
+  int part13 = 0;    // 13-qubit tiles: 0 = 512 threads, fragments streamed
+                     // through LDS, two workgroups per CU; 1 = 1024 threads,
+                     // fragments resident, next tile prefetched (when the
+                     // steps fit); 2 = as 0, each wave reading the fragments
+                     // from L2 straight into registers
 };
 
 // Applies the fused unitary of `gates` to the n-qubit state at `amp`

@@ -222,11 +222,13 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *                 (parts of 8..14 qubits; 0..64; 0, the default, = 8
  *                 where the next tile is prefetched during the steps, 1
  *                 where the U fragments are streamed);
- *   "part13_resident" 13-qubit tiles of the wide part kernel: 1 = 1024
- *                 threads, every step's U fragments in LDS, the next tile
+ *   "part13_u"    where 13-qubit tiles of the wide part kernel keep the
+ *                 steps' U fragments: 0 (the default) = 512 threads, U
+ *                 streamed through one LDS buffer, two workgroups per CU;
+ *                 1 = 1024 threads, every step's U in LDS, the next tile
  *                 prefetched into registers during the steps (parts of up
- *                 to 11 steps); 0 = 512 threads, U streamed, two
- *                 workgroups per CU;
+ *                 to 11 steps); 2 = as 0 but each wave reads U from L2
+ *                 straight into registers (no LDS buffer);
  *   "tile_order"  order in which the BF16 tile kernels' workgroups take
  *                 tiles: 1..3 = the highest 1..3 slots outside the tile vary
  *                 fastest, so concurrent tiles spread over the halves

@@ -231,13 +231,15 @@ def test_fused_chain_renormalize(orc, n, depth, k, dtype):
     assert np.abs(got - want).max() <= 1e-6
 
 
-@pytest.mark.parametrize("tpw", [0, 3])
+@pytest.mark.parametrize("form,tpw", [(1, 0), (1, 3), (2, 0), (2, 2)])
 @pytest.mark.parametrize("n,depth,k", [(16, 12, 13), (20, 12, 13), (19, 30, 13)])
-def test_fused_chain_part13_resident(orc, n, depth, k, tpw):
-    """The 13-qubit wide part kernel with its U fragments resident in LDS
-    (1024 threads, next tile prefetched during the steps; part13_resident=1),
-    with the default and a ragged number of tiles per workgroup: equal to the
-    streamed form within FP32 rounding and within 1e-6 of the FP64 oracle."""
+def test_fused_chain_part13_forms(orc, n, depth, k, form, tpw):
+    """The other forms of the 13-qubit wide part kernel (part13_u): 1 = U
+    fragments resident in LDS (1024 threads, next tile prefetched during the
+    steps), 2 = fragments read by each wave from L2 into registers; with the
+    default and a ragged number of tiles per workgroup: equal to the default
+    LDS-streamed form within FP32 rounding and within 1e-6 of the FP64
+    oracle."""
     from hisvsim_amd.partition import chain_parts
     gates = random_layered(n, depth, seed=11 * n + depth)
     parts, lows = chain_parts(gates, k)
@@ -245,13 +247,13 @@ def test_fused_chain_part13_resident(orc, n, depth, k, tpw):
     want = a.copy()
     orc.apply_gates(want, gates)
 
-    def run(resident):
+    def run(f):
         def fn(s):
-            s.tune("part13_resident", resident)
+            s.tune("part13_u", f)
             s.tune("part_tiles_per_wg", tpw)
             s.apply_fused_chain(parts, lows, renormalize=True)
         return gpu_run(n, a, fn)
-    got, streamed = run(1), run(0)
+    got, streamed = run(form), run(0)
     assert np.abs(got - streamed).max() <= 1e-6
     assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
     assert np.abs(got - want).max() <= 1e-6
