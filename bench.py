@@ -50,7 +50,7 @@ def log(msg: str) -> None:
 LOW_SLOTS = 5  # qubits the device adds to every LDS-staged part (libhsv default)
 
 
-def make_parts(gates, mode: str, k: int, partitioner: str = "best", layout: str = "fixed", beam=(8, 4)):
+def make_parts(gates, mode: str, k: int, partitioner: str = "best", layout: str = "fixed", beam=(16, 8)):
     """Acyclic parts: DAG-aware greedy, depth-first order, qubit-set
     closure, or whichever gives the fewest parts ("best"). For the register
     part kernel, k counts the low qubits the device adds to every part
@@ -220,7 +220,7 @@ def main() -> int:
                     help="acyclic partitioner (best: the one with fewer parts)")
     ap.add_argument("--pergate-layers", type=int, default=1,
                     help="layers of the per-gate timing sample (0: no sample)")
-    ap.add_argument("--beam", default="8,4", metavar="WIDTH,BRANCH",
+    ap.add_argument("--beam", default="16,8", metavar="WIDTH,BRANCH",
                     help="chain planner: beam search over the low sets (1,1 = greedy)")
     ap.add_argument("--layout", default="chain", choices=["chain", "fixed"],
                     help="wide BF16-split parts (k >= 8): moving low-slot layout (chain) or qubits 0-4 "

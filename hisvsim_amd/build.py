@@ -24,7 +24,7 @@ ARCH = os.environ.get("HSV_OFFLOAD_ARCH", "gfx950")
 # -packed-fp32-ops keeps the compiler from emitting packed FP32 VALU ops.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function", "-Wno-unused-value",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
-CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include")]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-I", os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["hsv.hip", "fused.hip", "qaoa.hip", "mlp.hip"]
 CXX_SOURCES = ["gates.cpp", "partition.cpp"]
@@ -50,7 +50,7 @@ def build(verbose: bool = True) -> dict:
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(_run, jobs))
     lib = os.path.join(HERE, "libhsv.so")
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", lib, *objs])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-pthread", "-o", lib, *objs])
 
     import pybind11  # build-time only
     ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"

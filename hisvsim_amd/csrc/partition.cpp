@@ -3,8 +3,10 @@
 // so that the lookahead over candidate low-slot sets can be wider at the same
 // planning time. Results equal the Python method's gate for gate.
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hsv.h"
@@ -126,11 +128,19 @@ struct ChainState {
   std::vector<uint64_t> lows;
 };
 
+// One beam state's next part and the ranking of the lows after it.
+struct Expansion {
+  std::vector<int> taken, keep;
+  uint64_t tile = 0;
+  std::vector<std::pair<int, uint64_t>> ranked;
+};
+
 // Beam search over the low-set choices: every state of the beam takes its
 // next part, each branches into its `branch` best-ranked low sets, and the
 // `width` children with the fewest gates left survive (duplicates dropped).
 // The first state to run out of gates ends the search. Width 1, branch 1 is
-// the greedy chain.
+// the greedy chain. The states of one round are expanded on host threads;
+// the result does not depend on their number.
 int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, int candidates, int width, int branch,
                ChainState* best) {
   const int num_gates = (int)qm.size();
@@ -140,12 +150,27 @@ int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, i
   s0.lowset = (1ull << low) - 1;
   s0.part_of_gate.assign(num_gates, 0);
   std::vector<ChainState> beam{s0};
-  std::vector<int> taken, keep;
+  const int hw = std::max(1, (int)std::thread::hardware_concurrency());
   while (true) {
+    std::vector<Expansion> ex(beam.size());
+    std::atomic<int> next{0};
+    auto work = [&]() {
+      for (int i = next++; i < (int)beam.size(); i = next++) {
+        Expansion& e = ex[i];
+        take_part(qm, beam[i].rem, beam[i].lowset, k, candidates, &e.taken, &e.keep, &e.tile);
+        if (!e.taken.empty() && !e.keep.empty()) e.ranked = rank_lows(qm, e.keep, e.tile, k, low, lookahead, candidates);
+      }
+    };
+    const int nthreads = std::min<int>(hw, (int)beam.size());
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+    work();
+    for (std::thread& t : pool) t.join();
     std::vector<std::pair<int, ChainState>> kids;  // (gates left after the child's next part, child)
-    for (const ChainState& s : beam) {
-      uint64_t tile = 0;
-      take_part(qm, s.rem, s.lowset, k, candidates, &taken, &keep, &tile);
+    for (size_t b = 0; b < beam.size(); ++b) {
+      const ChainState& s = beam[b];
+      const std::vector<int>& taken = ex[b].taken;
+      const std::vector<int>& keep = ex[b].keep;
       if (taken.empty()) continue;  // a gate wider than the part once its lows are in
       const int p = (int)s.lows.size();
       if (keep.empty()) {
@@ -155,7 +180,7 @@ int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, i
         best->rem.clear();
         return HSV_OK;
       }
-      const auto ranked = rank_lows(qm, keep, tile, k, low, lookahead, candidates);
+      const auto& ranked = ex[b].ranked;
       for (int j = 0; j < branch && j < (int)ranked.size(); ++j) {
         if (ranked[j].first == 0) break;
         ChainState c;
