@@ -11,7 +11,7 @@
         HBM bytes per dispatch of kernel K: its FETCH_SIZE and WRITE_SIZE (KiB
         per dispatch) scaled by bytes / counter-KiB measured on kernel C, a
         copy kernel that reads and writes B bytes each
-    python -m hisvsim_amd.profile_summary steps DIR [--n 33 --depth 40 --k 13]
+    python -m hisvsim_amd.profile_summary steps DIR [--n 33 --depth 40 --k 13 --beam 16,8]
         durations of the wide BF16-split part passes in a ``--kernel-trace``
         run of the bench default, grouped by the 5-qubit fused steps of each
         part (the bench's chain parts, regenerated here, in dispatch order)
@@ -114,7 +114,7 @@ def trace_durations(d: str, kernel: str, min_ms: float = 0.0) -> List[float]:
     return [row[0] for row in trace_dispatches(d, kernel) if row[0] >= min_ms]
 
 
-def steps_table(d: str, n: int, depth: int, k: int) -> dict:
+def steps_table(d: str, n: int, depth: int, k: int, beam=(16, 8)) -> dict:
     """Pass time of the first circuit's part passes by the number of 5-qubit
     steps in the part (the trace of one bench run of the chain). Passes over
     only part of the state (the first passes from |0...0>, which skip the
@@ -122,7 +122,7 @@ def steps_table(d: str, n: int, depth: int, k: int) -> dict:
     kernel instantiation) are left out."""
     from .circuit import random_layered
     from .partition import chain_parts, closure_parts
-    parts, _ = chain_parts(random_layered(n, depth, seed=0), k)
+    parts, _ = chain_parts(random_layered(n, depth, seed=0), k, beam=beam)
     disp = trace_dispatches(d, "k_part_bf")[:len(parts)]
     max_grid: Dict[str, int] = {}
     for _, grid, name in disp:
@@ -163,11 +163,13 @@ def main(argv=None) -> int:
     e.add_argument("--n", type=int, default=33)
     e.add_argument("--depth", type=int, default=40)
     e.add_argument("--k", type=int, default=13)
+    e.add_argument("--beam", default="16,8", help="the bench's chain planner beam (width,branch)")
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         print(stats_text(args.dir, args.top))
     elif args.cmd == "steps":
-        print(json.dumps(steps_table(args.dir, args.n, args.depth, args.k), indent=1))
+        beam = tuple(int(x) for x in args.beam.split(","))
+        print(json.dumps(steps_table(args.dir, args.n, args.depth, args.k, beam), indent=1))
     elif args.cmd == "counter":
         v = counter_values(args.dir, args.counter, args.kernel)
         print(json.dumps({"dispatches": len(v), "median": statistics.median(v) if v else None,
