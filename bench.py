@@ -350,10 +350,10 @@ def main() -> int:
                 "per_launch": f"{unit_bytes} B (2 x 16 B x 2^{n}: read + write of the state) / {kern_ms:.3f} ms"}
         if args.mode == "fusedbf16":
             # blocks per pass: 1 for k <= 7; for wide parts the 5-qubit steps the
-            # host cuts (the closure method of partition.closure_parts, k = 5)
+            # host cuts (hsv_part_steps)
             timed = ([p for p, f in zip(parts, chain_full_passes(parts, lows, n)) if f]
                      if lows is not None else parts)   # the passes kern_ms averages
-            blocks = ([len(closure_parts(p, 5)) for p in timed] if k >= 8 else [1] * len(timed))
+            blocks = ([len(capi.part_steps(p)) for p in timed] if k >= 8 else [1] * len(timed))
             width = 5 if k >= 8 else max(5, min(7, k))
             per_pass = float(np.mean(blocks)) * 8 * 2 ** width * 2 ** n
             roof["fused_blocks_per_pass"] = float(np.mean(blocks))

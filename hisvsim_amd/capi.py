@@ -93,6 +93,8 @@ SIGNATURES = {
     "hsv_chain_parts": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _I]),
+    "hsv_part_steps": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int32), _I]),
     "hsv_permute_qubits": (ctypes.c_int, [_P, _I]),
     "hsv_read": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
     "hsv_write": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64, _D]),
@@ -208,6 +210,21 @@ def chain_parts(gates: Sequence, k: int, low: int = 5, lookahead: int = 7, candi
     for g, pi in zip(gates, part[:len(gates)]):
         parts[pi].append(g)
     return parts, [list(map(int, lows[p * low:(p + 1) * low])) for p in range(num.value)]
+
+
+def part_steps(gates: Sequence, k: int = 5, width: int = 16):
+    """hsv_part_steps (host only): the gates of a wide BF16-split part in
+    the fused steps the kernel applies (identities left out)."""
+    gates = [g for g in gates if g.name.upper() != "I"]
+    masks = np.array([sum(1 << q for q in set(g.qubits())) for g in gates], dtype=np.uint64)
+    grp = np.zeros(max(1, len(gates)), np.int32)
+    num = ctypes.c_int(0)
+    _check(lib().hsv_part_steps(masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(gates), k, width,
+                                grp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(num)))
+    steps = [[] for _ in range(num.value)]
+    for g, s in zip(gates, grp[:len(gates)]):
+        steps[s].append(g)
+    return steps
 
 
 def fused_unitary(n: int, gates: Sequence):

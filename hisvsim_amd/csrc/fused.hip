@@ -26,6 +26,7 @@
 #include "../../include/hsv.h"
 #include "fused.hpp"
 #include "kernels.hpp"
+#include "partition.hpp"
 
 namespace hsv {
 
@@ -800,67 +801,6 @@ void build_tile_tables(uint64_t qmask, uint64_t cmask, BfTileTables* T) {
   }
 }
 
-// Qubit-set closure over gate qubit masks (the method of
-// hisvsim_amd/partition.py closure_parts): each group grows its qubit set by
-// the candidate that absorbs the most gates per added qubit, a gate being
-// absorbed when no earlier skipped gate shares a qubit with it. Groups run
-// in order; every qubit keeps its gate order.
-std::vector<std::vector<int>> closure_groups(const std::vector<uint64_t>& qm, int k) {
-  auto absorb = [&](const std::vector<int>& rem, uint64_t Q) {
-    std::vector<int> taken;
-    uint64_t blocked = 0;
-    for (int i : rem) {
-      const uint64_t g = qm[i];
-      if ((g & blocked) || (g & ~Q)) {
-        blocked |= g;
-        if ((Q & ~blocked) == 0) break;
-        continue;
-      }
-      taken.push_back(i);
-    }
-    return taken;
-  };
-  std::vector<int> rem(qm.size());
-  for (size_t i = 0; i < qm.size(); ++i) rem[i] = (int)i;
-  std::vector<std::vector<int>> out;
-  while (!rem.empty()) {
-    uint64_t Q = 0;
-    while (__builtin_popcountll(Q) < k) {
-      uint64_t blocked = 0;
-      std::vector<uint64_t> cands;
-      for (int i : rem) {
-        const uint64_t g = qm[i];
-        if ((g & ~Q) == 0) continue;
-        const uint64_t add = g & ~Q;
-        if (!(g & blocked) && __builtin_popcountll(Q | g) <= k &&
-            std::find(cands.begin(), cands.end(), add) == cands.end())
-          cands.push_back(add);
-        blocked |= g;
-        if (cands.size() >= 24) break;
-      }
-      if (cands.empty()) break;
-      uint64_t best = 0;
-      double best_score = -1;
-      for (uint64_t add : cands) {
-        const double sc = double(absorb(rem, Q | add).size()) / __builtin_popcountll(add);
-        if (sc > best_score) { best_score = sc; best = add; }
-      }
-      Q |= best;
-    }
-    std::vector<int> taken = absorb(rem, Q);
-    if (taken.empty()) return {};  // a gate wider than k
-    out.push_back(taken);
-    std::vector<int> keep;
-    size_t ti = 0;
-    for (int i : rem) {
-      if (ti < taken.size() && taken[ti] == i) { ++ti; continue; }
-      keep.push_back(i);
-    }
-    rem.swap(keep);
-  }
-  return out;
-}
-
 // LDS swizzle of k_part_bf. nimg[c] = h applied to tile bit c: 0 for
 // c < 4, a mask of bits 0-3 for c = 4, of bits 0-4 for c >= 5. A step is
 // conflict-free when the low-4-bit images of its 4 lowest column bits are
@@ -998,7 +938,8 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   bool moves = false;
   for (int j = 0; j < KT; ++j) moves = moves || sslot[j] != slot[j];
   if (qm.empty() && !moves) return HSV_OK;  // identities only
-  std::vector<std::vector<int>> groups = qm.empty() ? std::vector<std::vector<int>>() : closure_groups(qm, 5);
+  // the steps: groups of <= 5 qubits, as few as a beam of 16 finds
+  std::vector<std::vector<int>> groups = qm.empty() ? std::vector<std::vector<int>>() : step_groups(qm, 5, 16);
   if (groups.empty() && !qm.empty()) {
     *err = "a gate acts on more than 5 qubits: the wide bf16 fused path cannot take it";
     return HSV_ERR_LIMIT;

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/hsv.h"
+#include "partition.hpp"
 
 namespace {
 
@@ -206,6 +207,117 @@ int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, i
 }
 
 }  // namespace
+
+namespace hsv {
+
+std::vector<std::vector<int>> step_groups(const std::vector<uint64_t>& qm, int k, int width) {
+  const int ng = (int)qm.size();
+  std::vector<std::vector<int>> out;
+  if (ng == 0) return out;
+  uint64_t all = 0;
+  for (uint64_t g : qm) {
+    if (__builtin_popcountll(g) > k) return {};
+    all |= g;
+  }
+  // every k-subset of the part's qubits (all of them when fewer)
+  std::vector<int> qs;
+  for (int b = 0; b < 64; ++b)
+    if (all >> b & 1) qs.push_back(b);
+  const int n = (int)qs.size(), kk = std::min(k, n);
+  std::vector<uint64_t> subsets;
+  std::vector<int> idx(kk);
+  for (int i = 0; i < kk; ++i) idx[i] = i;
+  while (true) {
+    uint64_t c = 0;
+    for (int i : idx) c |= 1ull << qs[i];
+    subsets.push_back(c);
+    int i = kk - 1;
+    while (i >= 0 && idx[i] == n - kk + i) --i;
+    if (i < 0) break;
+    ++idx[i];
+    for (int j = i + 1; j < kk; ++j) idx[j] = idx[j - 1] + 1;
+  }
+  struct Grouping {
+    std::vector<int> rem;
+    std::vector<uint64_t> sets;  // the groups' qubit sets so far
+  };
+  Grouping g0;
+  g0.rem.resize(ng);
+  for (int i = 0; i < ng; ++i) g0.rem[i] = i;
+  std::vector<Grouping> beam{g0};
+  std::vector<int> taken, keep;
+  while (true) {
+    // (gates left, state, subset) for every extension that takes a gate
+    std::vector<std::pair<int, std::pair<int, int>>> ext;
+    for (int b = 0; b < (int)beam.size(); ++b)
+      for (int q = 0; q < (int)subsets.size(); ++q) {
+        const int got = absorb(qm, beam[b].rem, subsets[q], nullptr);
+        if (got > 0) ext.push_back({(int)beam[b].rem.size() - got, {b, q}});
+      }
+    if (ext.empty()) return {};
+    std::stable_sort(ext.begin(), ext.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::vector<Grouping> next;
+    for (const auto& e : ext) {
+      const Grouping& s = beam[e.second.first];
+      taken.clear();
+      absorb(qm, s.rem, subsets[e.second.second], &taken);
+      keep.clear();
+      size_t ti = 0;
+      for (int i : s.rem) {
+        if (ti < taken.size() && taken[ti] == i) {
+          ++ti;
+          continue;
+        }
+        keep.push_back(i);
+      }
+      Grouping c;
+      c.rem = keep;
+      c.sets = s.sets;
+      c.sets.push_back(subsets[e.second.second]);
+      if (keep.empty()) {
+        // replay the sets: the groups' gate lists
+        std::vector<int> rem = g0.rem;
+        for (uint64_t Q : c.sets) {
+          std::vector<int> t;
+          absorb(qm, rem, Q, &t);
+          out.push_back(t);
+          std::vector<int> kp;
+          size_t j = 0;
+          for (int i : rem) {
+            if (j < t.size() && t[j] == i) {
+              ++j;
+              continue;
+            }
+            kp.push_back(i);
+          }
+          rem.swap(kp);
+        }
+        return out;
+      }
+      bool dup = false;
+      for (const Grouping& x : next) dup = dup || x.rem == c.rem;
+      if (!dup) next.push_back(std::move(c));
+      if ((int)next.size() >= width) break;
+    }
+    beam.swap(next);
+  }
+}
+
+}  // namespace hsv
+
+extern "C" int hsv_part_steps(const uint64_t* gate_masks, int num_gates, int k, int width, int32_t* group_of_gate,
+                              int* num_groups) {
+  if (num_gates < 0 || (num_gates && (!gate_masks || !group_of_gate)) || !num_groups || k < 1 || k > 14 ||
+      width < 1)
+    return HSV_ERR_ARG;
+  const std::vector<uint64_t> qm(gate_masks, gate_masks + num_gates);
+  const auto groups = hsv::step_groups(qm, k, width);
+  if (num_gates && groups.empty()) return HSV_ERR_LIMIT;
+  for (size_t g = 0; g < groups.size(); ++g)
+    for (int i : groups[g]) group_of_gate[i] = (int32_t)g;
+  *num_groups = (int)groups.size();
+  return HSV_OK;
+}
 
 extern "C" int hsv_chain_parts(const uint64_t* gate_masks, int num_gates, int k, int low, int lookahead,
                                int candidates, int beam_width, int beam_branch, int32_t* part_of_gate,

@@ -121,7 +121,8 @@ def steps_table(d: str, n: int, depth: int, k: int, beam=(16, 8)) -> dict:
     tiles that are still zero: a smaller grid than the full passes of the
     kernel instantiation) are left out."""
     from .circuit import random_layered
-    from .partition import chain_parts, closure_parts
+    from . import capi
+    from .partition import chain_parts
     parts, _ = chain_parts(random_layered(n, depth, seed=0), k, beam=beam)
     disp = trace_dispatches(d, "k_part_bf")[:len(parts)]
     max_grid: Dict[str, int] = {}
@@ -133,7 +134,7 @@ def steps_table(d: str, n: int, depth: int, k: int, beam=(16, 8)) -> dict:
         if grid < max_grid[name]:
             partial += 1
             continue
-        by.setdefault(len(closure_parts(p, 5)), []).append(ms)
+        by.setdefault(len(capi.part_steps(p)), []).append(ms)
     every = trace_dispatches(d, "k_part_bf")
     for _, grid, name in every:
         max_grid[name] = max(max_grid.get(name, 0), grid)

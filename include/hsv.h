@@ -196,6 +196,16 @@ int hsv_gate_matrix(const hsv_gate* g, double* out, int* dim);
  * part per gate), *num_parts. HSV_ERR_LIMIT: a gate does not fit. */
 int hsv_chain_parts(const uint64_t* gate_masks, int num_gates, int k, int low, int lookahead, int candidates,
                     int beam_width, int beam_branch, int32_t* part_of_gate, int32_t* lows_out, int* num_parts);
+/* Host-only, not in the reference: the fused steps hsv_apply_fused and
+ * hsv_apply_fused_chain cut a wide BF16-split part into (k = 5, width 16).
+ * gate_masks[i] = the qubits of gate i (identities left out). Each step is
+ * the blocking scan of one k-qubit set (the gates, in order, inside it and
+ * behind no skipped gate on a shared qubit); a beam of `width` partial
+ * groupings over every k-subset of the part's qubits looks for the fewest
+ * steps. Out: group_of_gate[i], *num_groups. HSV_ERR_LIMIT: a gate has more
+ * than k qubits. */
+int hsv_part_steps(const uint64_t* gate_masks, int num_gates, int k, int width, int32_t* group_of_gate,
+                   int* num_groups);
 int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int* qubits_out,
                       int* num_part_qubits, double* u_out, double* unitarity_err);
 

@@ -243,6 +243,39 @@ def test_chain_parts_beam(n, depth, k, beam):
         chain_parts(gates, k, beam=beam, native=False)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_part_steps_are_a_valid_split(orc, seed):
+    """hsv_part_steps, the fused steps of a wide BF16-split part: every
+    step acts on at most 5 qubits, the steps keep each step's gates in
+    circuit order, applying them one after the other equals the part (oracle,
+    n = 14), and there are no more of them than the qubit-set closure makes."""
+    rng = np.random.default_rng(seed)
+    n = 14
+    gates = random_layered(n, 6, seed=seed)
+    parts, _ = chain_parts(gates, 12)
+    part = max(parts, key=len)
+    steps = capi.part_steps(part)
+    assert sum(len(s) for s in steps) == len(part)
+    pos = {id(g): i for i, g in enumerate(part)}
+    for s in steps:
+        assert len(set().union(*[set(g.qubits()) for g in s])) <= 5
+        assert [pos[id(g)] for g in s] == sorted(pos[id(g)] for g in s)
+    assert len(steps) <= len(closure_parts(part, 5))
+    a = rand_state(n, int(rng.integers(1 << 30)))
+    want, got = a.copy(), a.copy()
+    orc.apply_gates(want, part)
+    for s in steps:
+        orc.apply_gates(got, s)
+    assert np.abs(got - want).max() <= 1e-12
+
+
+def test_part_steps_limits():
+    assert capi.part_steps([]) == []
+    assert len(capi.part_steps([Gate("H", (q,)) for q in range(3)])) == 1
+    with pytest.raises(capi.HsvError, match="LIMIT"):
+        capi.part_steps([Gate("X", (0,), (1, 2, 3, 4, 5))])   # 6 qubits: wider than a step
+
+
 def test_chain_parts_apply_like_the_circuit(orc):
     """Applying the chain's parts in order equals the circuit (oracle, n=12)."""
     gates = random_layered(12, 8, seed=4)
