@@ -224,6 +224,10 @@ std::vector<std::vector<int>> step_groups(const std::vector<uint64_t>& qm, int k
   for (int b = 0; b < 64; ++b)
     if (all >> b & 1) qs.push_back(b);
   const int n = (int)qs.size(), kk = std::min(k, n);
+  // C(n, kk) candidate sets: a part of up to 14 qubits has at most 3432
+  double count = 1;
+  for (int i = 0; i < kk; ++i) count = count * (n - i) / (i + 1);
+  if (count > 65536) return {};
   std::vector<uint64_t> subsets;
   std::vector<int> idx(kk);
   for (int i = 0; i < kk; ++i) idx[i] = i;

@@ -13,7 +13,8 @@ namespace hsv {
 // a shared qubit), so every qubit keeps its gate order. Q ranges over every
 // k-subset of the part's qubits; a beam of `width` partial groupings (those
 // with the fewest gates left) searches for the fewest groups. Returns the
-// groups' gate indices; empty when a gate has more than k qubits.
+// groups' gate indices; empty when a gate has more than k qubits or the
+// part has more than 65536 k-subsets.
 std::vector<std::vector<int>> step_groups(const std::vector<uint64_t>& qm, int k, int width);
 
 }  // namespace hsv

@@ -203,7 +203,8 @@ int hsv_chain_parts(const uint64_t* gate_masks, int num_gates, int k, int low, i
  * behind no skipped gate on a shared qubit); a beam of `width` partial
  * groupings over every k-subset of the part's qubits looks for the fewest
  * steps. Out: group_of_gate[i], *num_groups. HSV_ERR_LIMIT: a gate has more
- * than k qubits. */
+ * than k qubits, or the gates span so many qubits that there are more than
+ * 65536 k-subsets (a wide part has at most 14). */
 int hsv_part_steps(const uint64_t* gate_masks, int num_gates, int k, int width, int32_t* group_of_gate,
                    int* num_groups);
 int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int* qubits_out,

@@ -274,6 +274,8 @@ def test_part_steps_limits():
     assert len(capi.part_steps([Gate("H", (q,)) for q in range(3)])) == 1
     with pytest.raises(capi.HsvError, match="LIMIT"):
         capi.part_steps([Gate("X", (0,), (1, 2, 3, 4, 5))])   # 6 qubits: wider than a step
+    with pytest.raises(capi.HsvError, match="LIMIT"):
+        capi.part_steps([Gate("H", (q,)) for q in range(40)])   # C(40, 5) candidate steps
 
 
 def test_chain_parts_apply_like_the_circuit(orc):
