@@ -1,7 +1,9 @@
 // Host-side planning for the moving-layout part chain (hsv_apply_fused_chain):
 // the method of hisvsim_amd/partition.py chain_parts, on 64-bit qubit masks,
 // so that the lookahead over candidate low-slot sets can be wider at the same
-// planning time. Results equal the Python method's gate for gate.
+// planning time (the greedy chain equals the Python method's gate for gate),
+// optionally widened into a beam search; and the grouping of a wide part's
+// gates into the 5-qubit fused steps of the BF16-split part kernel.
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
@@ -164,7 +166,11 @@ int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, i
     };
     const int nthreads = std::min<int>(hw, (int)beam.size());
     std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+    try {
+      for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+    } catch (...) {
+      // no more threads: the calling thread takes what is left
+    }
     work();
     for (std::thread& t : pool) t.join();
     std::vector<std::pair<int, ChainState>> kids;  // (gates left after the child's next part, child)
