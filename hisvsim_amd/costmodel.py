@@ -56,7 +56,7 @@ def part_features(part: Sequence[Gate], n: int, mode: str, low: Sequence[int] = 
     elif mode == "chain" or (mode == "fusedbf16" and k > 7):
         # the wide-part kernel's tile: 12 or 13 qubits holding slots 0-4
         width = 12 if len(set(qs) | (low5 if mode != "chain" else set())) <= 12 else 13
-        blocks = len(closure_parts(part, 5)) if part else 0
+        blocks = len(capi.part_steps(part)) if part else 0   # the kernel's fused steps
     else:
         width, blocks = max(4 if mode == "fused64" else 5, k), 1
     n1 = sum(1 for g in part if len(g.qubits()) == 1)

@@ -7,7 +7,8 @@
    generator) with n ~ U{20..30} qubits and depth ~ U{4..10}. Each runs under
    two plans drawn from the candidate set (costmodel.candidate_specs: fused
    FP64 / BF16-split unitaries, wide BF16-split parts, the moving-layout
-   chain, the register part kernel); every part's kernel time is recorded
+   chain, the register part kernel), from a state taken off |0...0> (so
+   every pass covers the whole state); every part's kernel time is recorded
    with HIP events, and a chain's layout restore on its own.
 2. Training: the 3-layer MLP (64 hidden units, FP32) on log(ms per 2^30
    amplitudes), by libhsv's own forward/backward/Adam kernels; 10% held out.
@@ -28,13 +29,22 @@ import time
 import numpy as np
 
 from . import capi
-from .circuit import random_layered
+from .circuit import Gate, random_layered
 from .costmodel import (MLP, PARAMS_FILE, RESTORE_MS_PER_2_30, build_plan, candidate_specs, choose_plan, fallback_plan,
                         ms_of, plan_features, predict_plan_ms, run_plan, target_of)
 
 
 def log(msg):
     print(msg, flush=True)
+
+
+def off_basis(s) -> None:
+    """Resets the state and takes it off |0...0> with one untimed gate, so
+    that every pass of a plan runs over the whole state (from |0...0> the
+    chain's first passes skip the tiles that are still zero, which would
+    teach the model pass times that no feature explains)."""
+    s.reset()
+    s.apply_gate(Gate("H", (s.n - 1,)))
 
 
 def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10), max_seconds: float = 0.0):
@@ -64,7 +74,7 @@ def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10), max_seco
             s = capi.State(n)
         for pi in picks:
             plan = build_plan(gates, n, specs[pi])
-            s.reset()
+            off_basis(s)
             times = run_plan(s, plan)
             for f, ms in zip(plan_features(plan, n), times):
                 X.append(f)
@@ -101,7 +111,7 @@ def evaluate(model: MLP, num: int, seed: int, n: int = 32):
             fb["predicted_ms"] = predict_plan_ms(model, fb, n)
             res = {"depth": depth, "gates": len(gates)}
             for name, plan in (("model", best), ("fallback", fb)):
-                s.reset()
+                off_basis(s)
                 t0 = time.perf_counter()
                 times = run_plan(s, plan)
                 wall = time.perf_counter() - t0
