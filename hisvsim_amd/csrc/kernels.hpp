@@ -443,7 +443,7 @@ struct SwapTiles {
   int npairs;                    // pairs outside B
   uint8_t pa[32], pb[32];
 };
-constexpr int kSwapThreads = 256;
+constexpr int kSwapThreads = 512;
 constexpr int kSwapPer = 1024 / kSwapThreads;  // positions per thread and tile (nb <= 10)
 // Optionally fused with a renormalisation: norm_out (one FP64 partial of
 // sum |a|^2 per workgroup, in a fixed order) in one pass, scale (a device
