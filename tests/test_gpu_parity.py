@@ -259,6 +259,25 @@ def test_fused_chain_part13_forms(orc, n, depth, k, form, tpw):
     assert np.abs(got - want).max() <= 1e-6
 
 
+@pytest.mark.parametrize("n,depth,beam", [(20, 14, (4, 4)), (22, 12, (16, 8))])
+def test_fused_chain_beam_plan(orc, n, depth, beam):
+    """The bench's plan: the chain planned with a beam over the low sets
+    (hsv_chain_parts) and run from |0...0> with the folded renormalisation;
+    within 1e-6 of the FP64 oracle, unit norm."""
+    from hisvsim_amd.partition import chain_parts
+    gates = random_layered(n, depth, seed=3 * n + depth)
+    parts, lows = chain_parts(gates, 13, beam=beam)
+    want = np.zeros(2 ** n, complex)
+    want[0] = 1
+    orc.apply_gates(want, gates)
+    with capi.State(n) as s:
+        s.apply_fused_chain(parts, lows, renormalize=True)
+        got = s.read()
+    assert abs(np.vdot(got, got).real - 1) <= 1e-12
+    assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
+    assert np.abs(got - want).max() <= 1e-6
+
+
 def test_fused_chain_renormalize_zero_state():
     """A zero state stays zero (no error) under the renormalising chain."""
     from hisvsim_amd.partition import chain_parts
