@@ -233,8 +233,8 @@ def main() -> int:
                     help="skip the full-size infidelity check of a fused mode against the FP64 part path")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU layout is described "
-              f"in DESIGN.md and not run here.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): sharding the state over several "
+              f"GPUs is a design, not built or run here.", file=sys.stderr)
         return 2
     k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 13, "pergate": 1, "permute": 3}[args.mode]
     n, depth = args.n or (32 if args.mode == "permute" else 33), args.depth
