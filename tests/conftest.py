@@ -33,3 +33,22 @@ def bit_reverse_perm(n: int) -> np.ndarray:
     for b in range(n):
         rev |= ((idx >> b) & 1) << (n - 1 - b)
     return rev
+
+
+def layered_inverse(gates):
+    """The inverse of a random layered circuit (U3 and CZ gates): the gates
+    in reverse order, U3(theta, phi, lam) undone by U3(-theta, -lam, -phi)
+    (in the reference's convention, U3 = exp(-i (phi + lam) / 2) times the
+    textbook matrix, this is the exact inverse, phase included), CZ by
+    itself."""
+    from hisvsim_amd.circuit import Gate
+    out = []
+    for g in reversed(gates):
+        if g.name == "U3":
+            th, ph, lam = g.params
+            out.append(Gate("U3", g.targets, g.controls, (-th, -lam, -ph)))
+        elif g.name == "CZ" or (g.name == "Z" and len(g.controls) == 1):
+            out.append(g)
+        else:
+            raise ValueError(f"no inverse for {g.name}")
+    return out

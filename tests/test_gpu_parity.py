@@ -278,6 +278,24 @@ def test_fused_chain_beam_plan(orc, n, depth, beam):
     assert np.abs(got - want).max() <= 1e-6
 
 
+@pytest.mark.parametrize("n,dtype", [(30, "complex128"), (31, "complex64")])
+def test_fused_chain_round_trip_full_size(n, dtype):
+    """A size-independent check at 17 GB states: a random layered circuit
+    followed by its exact inverse, planned as one beam chain and run from
+    |0...0> with the folded renormalisation, must come back to |0...0>:
+    |<0|psi>|^2 >= 1 - 1e-6 (the BF16-split bar) and unit norm."""
+    from conftest import layered_inverse
+    from hisvsim_amd.partition import chain_parts
+    gates = random_layered(n, 12, seed=n)
+    parts, lows = chain_parts(gates + layered_inverse(gates), 13, beam=(4, 4))
+    with capi.State(n, dtype) as s:
+        s.apply_fused_chain(parts, lows, renormalize=True)
+        a0 = complex(s.read(0, 1)[0])
+        norm2 = s.norm2()
+    assert abs(a0) ** 2 >= 1 - 1e-6, (a0, n, dtype)
+    assert abs(norm2 - 1) <= (1e-12 if dtype == "complex128" else 1e-5)
+
+
 def test_fused_chain_renormalize_zero_state():
     """A zero state stays zero (no error) under the renormalising chain."""
     from hisvsim_amd.partition import chain_parts

@@ -290,6 +290,20 @@ def test_chain_parts_apply_like_the_circuit(orc):
     assert np.abs(got - want).max() <= 1e-12
 
 
+def test_layered_inverse_undoes_the_circuit(orc):
+    """The inverse the full-size round-trip test uses: circuit then inverse
+    is the identity on a random state (oracle, n = 10, 1e-12)."""
+    from conftest import layered_inverse
+    gates = random_layered(10, 6, seed=9)
+    a = rand_state(10, 3)
+    got = a.copy()
+    orc.apply_gates(got, gates + layered_inverse(gates))
+    assert np.abs(got - a).max() <= 1e-12
+    half = a.copy()
+    orc.apply_gates(half, gates)
+    assert np.abs(half - a).max() > 0.1   # the circuit itself is far from the identity
+
+
 def test_check_parts_detects_reordering():
     gates = [Gate("H", (0,)), Gate("X", (1,), (0,)), Gate("H", (0,))]
     with pytest.raises(ValueError):
