@@ -278,9 +278,10 @@ def test_fused_chain_beam_plan(orc, n, depth, beam):
     assert np.abs(got - want).max() <= 1e-6
 
 
-@pytest.mark.parametrize("n,dtype", [(30, "complex128"), (31, "complex64")])
+@pytest.mark.parametrize("n,dtype", [(30, "complex128"), (31, "complex64"), (33, "complex128"), (34, "complex64")])
 def test_fused_chain_round_trip_full_size(n, dtype):
-    """A size-independent check at 17 GB states: a random layered circuit
+    """A size-independent check at 17 GB and at 137 GB states (33 qubits
+    complex128 and 34 complex64, the benchmark sizes): a random layered circuit
     followed by its exact inverse, planned as one beam chain and run from
     |0...0> with the folded renormalisation, must come back to |0...0>:
     |<0|psi>|^2 >= 1 - 1e-6 (the BF16-split bar) and unit norm."""
