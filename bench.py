@@ -5,7 +5,8 @@ beside it.
 
     python bench.py [--steps K] [--warmup W] [--n 33] [--depth 40]
                     [--mode fusedbf16|part|fused64|fused32|pergate|permute] [--k K]
-                    [--layout chain|fixed] [--tune KEY=VALUE ...]
+                    [--layout chain|fixed] [--beam W,B] [--dtype complex128|complex64]
+                    [--tune KEY=VALUE ...]
 
 One step = one full simulation of the circuit from |0...0> (state reset
 included) with the chosen path; the state (2^n amplitudes) stays in HBM. The
@@ -146,7 +147,7 @@ def run_path(s: capi.State, mode: str, gates, parts, lows=None) -> None:
 
 
 def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0, partitioner: str = "best",
-                 layout: str = "fixed") -> dict:
+                 layout: str = "fixed", dtype: str = "complex128") -> dict:
     """Infidelity and max |d amp| of the chosen path against the FP64 oracle."""
     from oracle.oracle import Oracle
     if mode == "permute":
@@ -166,7 +167,7 @@ def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0, part
     want[0] = 1
     o = Oracle()
     o.apply_gates(want, gates)
-    with capi.State(n) as s:
+    with capi.State(n, dtype) as s:
         run_path(s, mode, gates, parts, lows)
         got = s.read()
     infid = max(0.0, 1.0 - abs(np.vdot(want, got)) ** 2)
@@ -220,6 +221,8 @@ def main() -> int:
                     help="acyclic partitioner (best: the one with fewer parts)")
     ap.add_argument("--pergate-layers", type=int, default=1,
                     help="layers of the per-gate timing sample (0: no sample)")
+    ap.add_argument("--dtype", default="complex128", choices=["complex128", "complex64"],
+                    help="state storage (complex64: the 34-qubit, 137 GB configuration with --n 34)")
     ap.add_argument("--beam", default="16,8", metavar="WIDTH,BRANCH",
                     help="chain planner: beam search over the low sets (1,1 = greedy)")
     ap.add_argument("--layout", default="chain", choices=["chain", "fixed"],
@@ -247,12 +250,12 @@ def main() -> int:
     else:
         beam = tuple(int(x) for x in args.beam.split(","))
         parts, partitioner_used, lows = make_parts(gates, args.mode, k, args.partitioner, args.layout, beam)
-    S = 16
+    S = 16 if args.dtype == "complex128" else 8
     total_upd = sum(gate_amplitudes(g, n) for g in gates)
     log(f"[bench] n={n} depth={depth} gates={len(gates)} mode={args.mode} k={k} parts={len(parts)}")
 
     tune = [(kv.split("=")[0], int(kv.split("=")[1])) for kv in args.tune]
-    with capi.State(n) as s:
+    with capi.State(n, args.dtype) as s:
         for key, value in tune:
             s.tune(key, value)
         for w in range(args.warmup):
@@ -293,7 +296,7 @@ def main() -> int:
             + (f", {partial['passes']} partial passes from |0...0> in {partial['ms']:.1f} ms" if partial else ""))
 
         full_verify = None
-        if args.mode in FUSED_MODE and not args.no_full_verify:
+        if args.mode in FUSED_MODE and not args.no_full_verify and args.dtype == "complex128":
             # the FP64 register part path on a second full-size state (two
             # 2^33 complex128 states fit in HBM together): infidelity of the
             # timed result at the benchmark's own size
@@ -347,7 +350,7 @@ def main() -> int:
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, n, args.mode, k),
                 "kernel": kernel,
-                "per_launch": f"{unit_bytes} B (2 x 16 B x 2^{n}: read + write of the state) / {kern_ms:.3f} ms"}
+                "per_launch": f"{unit_bytes} B (2 x {S} B x 2^{n}: read + write of the state) / {kern_ms:.3f} ms"}
         if args.mode == "fusedbf16":
             # blocks per pass: 1 for k <= 7; for wide parts the 5-qubit steps the
             # host cuts (hsv_part_steps)
@@ -371,7 +374,7 @@ def main() -> int:
                 "hbm_gbs": 2 * S * 2 ** n / (kern_ms * 1e-3) / 1e9}
 
     verify = None if args.no_verify else verify_small(args.mode, k, depth, partitioner=args.partitioner,
-                                                      layout=args.layout)
+                                                      layout=args.layout, dtype=args.dtype)
     if verify:
         log(f"[bench] verify vs FP64 oracle at n={verify['n']}: infidelity {verify['infidelity']}, "
             f"max|d| {verify['max_abs_err']:.2e}")
@@ -384,8 +387,8 @@ def main() -> int:
         "ms_per_step": tts * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": {"fused32": "f32split", "fusedbf16": "bf16split (fp32 accumulate)"}.get(args.mode, "f64"),
         "data": "synthetic: seeded random layered circuit (U3 angles ~ U[0,2pi), CZ on random matchings), |0..0> input",
-        "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, complex128 state "
-                                f"({2 ** n * 16 / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
+        "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, {args.dtype} state "
+                                f"({2 ** n * S / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
                     "mode": args.mode, "k": k, "passes": passes, "partitioner": partitioner_used if lows is None else f"chain (beam {beam[0]}x{beam[1]})",
                     **({"layout": "chain (moving low slots, one restoring permutation pass)",
                         "layout_restore_ms": restore_ms, "partial_passes_from_zero": partial}
