@@ -14,6 +14,10 @@
 #include "../../include/hsv.h"
 #include "partition.hpp"
 
+namespace hsv {
+std::vector<std::vector<int>> step_groups(const std::vector<uint64_t>& qm, int k, int width);
+}
+
 namespace {
 
 // The blocking scan for a fixed qubit set Q: the remaining gates, in order,
@@ -129,21 +133,36 @@ struct ChainState {
   uint64_t lowset;
   std::vector<int32_t> part_of_gate;
   std::vector<uint64_t> lows;
+  int steps = 0;  // fused 5-qubit steps of the parts so far (beam only)
 };
 
-// One beam state's next part and the ranking of the lows after it.
+// One beam state's next part, its fused steps and the ranking of the lows
+// after it.
 struct Expansion {
   std::vector<int> taken, keep;
   uint64_t tile = 0;
+  int steps = 0;
   std::vector<std::pair<int, uint64_t>> ranked;
 };
+
+// The fused steps the wide part kernel cuts a part into (as hsv_part_steps).
+int part_step_count(const std::vector<uint64_t>& qm, const std::vector<int>& taken) {
+  std::vector<uint64_t> m;
+  m.reserve(taken.size());
+  for (int i : taken) m.push_back(qm[i]);
+  const auto g = hsv::step_groups(m, 5, 16);
+  return g.empty() ? (int)taken.size() : (int)g.size();
+}
 
 // Beam search over the low-set choices: every state of the beam takes its
 // next part, each branches into its `branch` best-ranked low sets, and the
 // `width` children with the fewest gates left survive (duplicates dropped).
-// The first state to run out of gates ends the search. Width 1, branch 1 is
-// the greedy chain. The states of one round are expanded on host threads;
-// the result does not depend on their number.
+// The round in which states run out of gates ends the search, with the one
+// whose parts cut into the fewest fused steps (ranking the children by steps
+// as well measured worse: it lets plans with one more part through). Width
+// 1, branch 1 is the greedy chain (no step counts). The
+// states of one round are expanded on host threads; the result does not
+// depend on their number.
 int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, int candidates, int width, int branch,
                ChainState* best) {
   const int num_gates = (int)qm.size();
@@ -154,6 +173,7 @@ int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, i
   s0.part_of_gate.assign(num_gates, 0);
   std::vector<ChainState> beam{s0};
   const int hw = std::max(1, (int)std::thread::hardware_concurrency());
+  const bool steered = width > 1 || branch > 1;
   while (true) {
     std::vector<Expansion> ex(beam.size());
     std::atomic<int> next{0};
@@ -162,6 +182,7 @@ int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, i
         Expansion& e = ex[i];
         take_part(qm, beam[i].rem, beam[i].lowset, k, candidates, &e.taken, &e.keep, &e.tile);
         if (!e.taken.empty() && !e.keep.empty()) e.ranked = rank_lows(qm, e.keep, e.tile, k, low, lookahead, candidates);
+        if (steered && !e.taken.empty()) e.steps = part_step_count(qm, e.taken);
       }
     };
     const int nthreads = std::min<int>(hw, (int)beam.size());
@@ -174,19 +195,26 @@ int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, i
     work();
     for (std::thread& t : pool) t.join();
     std::vector<std::pair<int, ChainState>> kids;  // (gates left after the child's next part, child)
+    int done = -1;  // the finished state of fewest steps, this round
+    for (size_t b = 0; b < beam.size(); ++b)
+      if (!ex[b].taken.empty() && ex[b].keep.empty() &&
+          (done < 0 || beam[b].steps + ex[b].steps < beam[done].steps + ex[done].steps))
+        done = (int)b;
+    if (done >= 0) {
+      const ChainState& s = beam[done];
+      *best = s;
+      for (int i : ex[done].taken) best->part_of_gate[i] = (int)s.lows.size();
+      best->lows.push_back(s.lowset);
+      best->steps = s.steps + ex[done].steps;
+      best->rem.clear();
+      return HSV_OK;
+    }
     for (size_t b = 0; b < beam.size(); ++b) {
       const ChainState& s = beam[b];
       const std::vector<int>& taken = ex[b].taken;
       const std::vector<int>& keep = ex[b].keep;
       if (taken.empty()) continue;  // a gate wider than the part once its lows are in
       const int p = (int)s.lows.size();
-      if (keep.empty()) {
-        *best = s;
-        for (int i : taken) best->part_of_gate[i] = p;
-        best->lows.push_back(s.lowset);
-        best->rem.clear();
-        return HSV_OK;
-      }
       const auto& ranked = ex[b].ranked;
       for (int j = 0; j < branch && j < (int)ranked.size(); ++j) {
         if (ranked[j].first == 0) break;
@@ -197,6 +225,7 @@ int chain_beam(const std::vector<uint64_t>& qm, int k, int low, int lookahead, i
         for (int i : taken) c.part_of_gate[i] = p;
         c.lows = s.lows;
         c.lows.push_back(s.lowset);
+        c.steps = s.steps + ex[b].steps;
         kids.push_back({(int)keep.size() - ranked[j].first, std::move(c)});
       }
     }
@@ -347,10 +376,16 @@ extern "C" int hsv_chain_parts(const uint64_t* gate_masks, int num_gates, int k,
   if (int rc = chain_beam(qm, k, low, lookahead, candidates, 1, 1, &best)) return rc;
   if (beam_width > 1 || beam_branch > 1) {
     ChainState wide;
-    // the beam can end worse than the greedy chain: keep the shorter
-    if (chain_beam(qm, k, low, lookahead, candidates, beam_width, beam_branch, &wide) == HSV_OK &&
-        wide.lows.size() < best.lows.size())
-      best = std::move(wide);
+    // the beam can end worse than the greedy chain: keep the shorter, or at
+    // equal length the one of fewer fused steps
+    if (chain_beam(qm, k, low, lookahead, candidates, beam_width, beam_branch, &wide) == HSV_OK) {
+      std::vector<std::vector<int>> gp(best.lows.size());
+      for (int i = 0; i < num_gates; ++i) gp[best.part_of_gate[i]].push_back(i);
+      best.steps = 0;
+      for (const auto& t : gp) best.steps += part_step_count(qm, t);
+      if (wide.lows.size() < best.lows.size() || (wide.lows.size() == best.lows.size() && wide.steps < best.steps))
+        best = std::move(wide);
+    }
   }
   for (int i = 0; i < num_gates; ++i) part_of_gate[i] = best.part_of_gate[i];
   for (size_t p = 0; p < best.lows.size(); ++p)
