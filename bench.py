@@ -111,15 +111,37 @@ def exchange_permutation(n: int, m: int = 3):
 
 
 def chain_full_passes(parts, lows, n: int):
-    """Which chain passes run over the whole state: starting from |0...0>,
-    hsv_apply_fused_chain skips the tiles with a 1 in a qubit no earlier
-    part has touched, so a pass is full once the touched qubits and the
-    part's own tile cover all n."""
+    """Which chain passes run over the whole state. Starting from |0...0>,
+    hsv_apply_fused_chain skips the tiles with a 1 in a slot whose qubit no
+    earlier part has touched; a pass is full when every slot outside its
+    tile holds a touched qubit. The tile is part_apply_bf's: the slots of
+    the part's qubits (identities left out) and its lows, padded with the
+    lowest free slots to at least 12 (13 when slots 0-4 are not all in it).
+    The slot layout follows the chain's own rule: each next low not already
+    in slots 0-4 swaps slots with the last current low the next part does
+    not list."""
+    phys = list(range(n))
     touched, full = set(), []
-    for p, low in zip(parts, lows):
-        q = {x for g in p for x in g.qubits()}
-        full.append(len(touched | q | set(low)) >= n)
+    for p, (part, low) in enumerate(zip(parts, lows)):
+        q = {x for g in part if g.name.upper() != "I" for x in g.qubits()}
+        pmask = {phys[x] for x in q | set(low)}
+        kt = max(12, min(14, len(pmask | set(range(min(n, 5))))))
+        tmask = set(pmask)
+        for b in range(n):
+            if len(tmask) >= kt:
+                break
+            tmask.add(b)
+        live = {phys[x] for x in touched}
+        full.append(all(b in live for b in range(n) if b not in tmask))
         touched |= q
+        if p + 1 < len(parts):
+            want = set(lows[p + 1])
+            spare = [x for x in low if x not in want]
+            for x in lows[p + 1]:
+                if phys[x] < 5:
+                    continue
+                o = spare.pop()
+                phys[x], phys[o] = phys[o], phys[x]
     return full
 
 
@@ -174,24 +196,34 @@ def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0, part
     return {"n": n, "depth": depth, "infidelity": infid, "max_abs_err": float(np.abs(got - want).max())}
 
 
-def cpu_baseline(n_full: int, depth_full: int, k: int, seconds: float = 20.0) -> dict:
-    """The reference (oracle/_ref) timed on the host cores on a bounded sample:
-    the first layers of the same circuit generator at a smaller n, scaled to
-    the full circuit by amplitude updates."""
+def _runs(mask: int) -> int:
+    """Contiguous runs of set bits (the reference's loop() nests one level
+    per run and indexes past its tables beyond 10, loop.hpp:922-960)."""
+    return bin(mask & ~(mask << 1)).count("1")
+
+
+def cpu_baseline(n_full: int, depth_full: int, k: int, seconds: float = 12.0) -> dict:
+    """The reference build (oracle/_ref) timed on the host cores on bounded
+    samples of the same circuit generator, each scaled to the full circuit:
+
+    * its partitioned hot path, gather_and_execute_on_multi
+      (execute.hpp:318-470) with opt_slots 1 on parts of <= 10 qubits from
+      the qubit-set closure partitioner, timed on the first parts of the
+      circuit at 28 qubits and scaled per part pass and amplitude to the
+      full circuit's parts (this is `value`);
+    * its per-gate path, StateVector::apply (state_vector.hpp:402-510), on
+      the first layers at 26 qubits, scaled by gate-amplitude updates."""
     from oracle.oracle import RefSim
     if not RefSim.available():
         return {"value": None, "unit": "s", "cores": None, "kind": "reference",
                 "sample": "oracle/_ref/libref.so not present"}
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 64)
+    cores = min(len(os.sched_getaffinity(0)), 64)
     ref = RefSim()
     ref.set_threads(cores)
+    # per-gate path
     n = min(n_full, 26)
-    layers = 1
-    gates = random_layered(n, layers, seed=0)
-    upd = sum(gate_amplitudes(g, n) for g in gates)
+    gates = random_layered(n, 1, seed=0)
     t = ref.time_from_zero(n, gates=gates)
-    # scale the sample to ~`seconds` of CPU work
     layers = max(1, min(depth_full, int(seconds / max(t, 1e-3))))
     gates = random_layered(n, layers, seed=0)
     upd = sum(gate_amplitudes(g, n) for g in gates)
@@ -199,11 +231,40 @@ def cpu_baseline(n_full: int, depth_full: int, k: int, seconds: float = 20.0) ->
     rate = upd / t
     full_gates = random_layered(n_full, depth_full, seed=0)
     full_upd = sum(gate_amplitudes(g, n_full) for g in full_gates)
-    return {"value": full_upd / rate, "unit": "s", "cores": cores, "kind": "reference",
-            "sample": f"reference per-gate path (StateVector::apply, OpenMP, {cores} threads), "
-                      f"{n}-qubit {layers}-layer prefix of the same generator ({len(gates)} gates, {t:.1f} s); "
-                      f"{rate:.3e} gate-amplitude updates/s, extrapolated to {n_full}q depth {depth_full}",
-            "gate_amp_updates_per_s": rate}
+    per_gate = {"value": full_upd / rate, "unit": "s", "n_sample": n, "layers": layers, "gates": len(gates),
+                "sample_s": t, "gate_amp_updates_per_s": rate}
+    # partitioned path: parts the reference's loop() accepts (<= 10 runs
+    # inside and outside the part, qubits 0-3 added by opt_slots)
+    kp, np_ = 10, min(n_full, 28)
+    all_np = (1 << np_) - 1
+
+    def accepted(part):
+        m = 0xF
+        for g in part:
+            for q in g.qubits():
+                m |= 1 << q
+        return _runs(m) <= 10 and _runs(all_np & ~m) <= 10
+    parts = [p for p in closure_parts(random_layered(np_, depth_full, seed=0), kp) if accepted(p)]
+    t1 = ref.time_from_zero(np_, parts=parts[:1])
+    m = max(1, min(len(parts), int(seconds / max(t1, 1e-3))))
+    # m parts spread evenly over the circuit (the timing does not depend on
+    # the amplitudes, only on each part's qubits and gates)
+    sample = [parts[i] for i in sorted(set(np.linspace(0, len(parts) - 1, m).astype(int)))]
+    m = len(sample)
+    tp = ref.time_from_zero(np_, parts=sample)
+    per_pass_amp = tp / (m * 2 ** np_)
+    full_parts = len(closure_parts(full_gates, kp))
+    value = per_pass_amp * full_parts * 2 ** n_full
+    return {"value": value, "unit": "s", "cores": cores, "kind": "reference",
+            "sample": f"reference partitioned path (gather_and_execute_on_multi, opt_slots 1, OpenMP, {cores} "
+                      f"threads) on {m} closure parts (<= {kp} qubits, spread over the circuit) of the same generator at "
+                      f"{np_} qubits ({tp:.1f} s, {per_pass_amp * 1e9:.2f} ns per amplitude and part pass), "
+                      f"extrapolated to the {full_parts} parts of {n_full}q depth {depth_full}; per-gate path "
+                      f"(StateVector::apply) beside it: {n}-qubit {layers}-layer prefix ({t:.1f} s), "
+                      f"extrapolated by gate-amplitude updates to {per_gate['value']:.0f} s",
+            "partitioned": {"value": value, "n_sample": np_, "k": kp, "parts_timed": m, "sample_s": tp,
+                            "s_per_amp_pass": per_pass_amp, "full_parts": full_parts},
+            "per_gate": per_gate}
 
 
 def main() -> int:
@@ -242,14 +303,17 @@ def main() -> int:
     k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 13, "pergate": 1, "permute": 3}[args.mode]
     n, depth = args.n or (32 if args.mode == "permute" else 33), args.depth
     gates = random_layered(n, depth, seed=0)
-    partitioner_used, lows = None, None
+    partitioner_used, lows, plan_s = None, None, None
     if args.mode == "permute":
         parts = exchange_permutation(n, k)
     elif args.mode == "pergate":
         parts = []
     else:
         beam = tuple(int(x) for x in args.beam.split(","))
+        tp = time.perf_counter()
         parts, partitioner_used, lows = make_parts(gates, args.mode, k, args.partitioner, args.layout, beam)
+        plan_s = time.perf_counter() - tp
+        log(f"[bench] planning ({partitioner_used}) took {plan_s:.2f} s on the host")
     S = 16 if args.dtype == "complex128" else 8
     total_upd = sum(gate_amplitudes(g, n) for g in gates)
     log(f"[bench] n={n} depth={depth} gates={len(gates)} mode={args.mode} k={k} parts={len(parts)}")
@@ -278,11 +342,15 @@ def main() -> int:
         per_step = len(times) // max(1, args.steps)
         launches = times[: per_step * args.steps].reshape(args.steps, per_step)
         restore_ms, partial = None, None
-        if lows is not None and per_step == len(parts) + 1:
-            # the moving layout ends with one permutation pass back to the
-            # standard layout: reported on its own, not in the part average
-            restore_ms = float(launches[:, -1].mean())
-            launches = launches[:, :-1]
+        if lows is not None:
+            if per_step not in (len(parts), len(parts) + 1):
+                raise RuntimeError(f"{per_step} timed launches per step for {len(parts)} chain parts")
+            if per_step == len(parts) + 1:
+                # the moving layout ends with one timed restore (a tile pass,
+                # or the two swap passes) back to the standard layout:
+                # reported on its own, not in the part average
+                restore_ms = float(launches[:, -1].mean())
+                launches = launches[:, :-1]
             # from |0...0> the chain's first passes skip the tiles that are
             # still zero: they move less than the whole state, so the
             # per-pass rate below counts the full passes only
@@ -390,6 +458,10 @@ def main() -> int:
         "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, {args.dtype} state "
                                 f"({2 ** n * S / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
                     "mode": args.mode, "k": k, "passes": passes, "partitioner": partitioner_used if lows is None else f"chain (beam {beam[0]}x{beam[1]})",
+                    # host planning runs once per circuit before the timed
+                    # steps, as the reference reads a precomputed parts file:
+                    # `value` excludes it
+                    "plan_s": plan_s, "plan_host_threads": len(os.sched_getaffinity(0)),
                     **({"layout": "chain (moving low slots, one restoring permutation pass)",
                         "layout_restore_ms": restore_ms, "partial_passes_from_zero": partial}
                        if lows is not None else {}),

@@ -197,8 +197,11 @@ def gate_matrix(g) -> np.ndarray:
 def chain_parts(gates: Sequence, k: int, low: int = 5, lookahead: int = 7, candidates: int = 24,
                 beam: Tuple[int, int] = (1, 1)):
     """hsv_chain_parts (host only): (parts, lows) as partition.chain_parts;
-    beam = (width, branch) of the search over low sets, (1, 1) = greedy."""
-    masks = np.array([sum(1 << q for q in set(g.qubits())) for g in gates], dtype=np.uint64)
+    beam = (width, branch) of the search over low sets, (1, 1) = greedy.
+    Identity gates get an empty mask: the chain leaves them out of a part's
+    tile."""
+    masks = np.array([0 if g.name.upper() == "I" else sum(1 << q for q in set(g.qubits())) for g in gates],
+                     dtype=np.uint64)
     part = np.zeros(max(1, len(gates)), np.int32)
     lows = np.zeros(max(1, len(gates)) * low + low, np.int32)
     num = ctypes.c_int(0)
@@ -340,6 +343,18 @@ class State:
         v = ctypes.c_double()
         _check(lib().hsv_max_abs_diff(self._h, other._h, ctypes.byref(v)))
         return v.value
+
+    def max_abs(self) -> float:
+        """max |a_i| (FP64 reduction on the device)."""
+        v = ctypes.c_double()
+        _check(lib().hsv_max_abs_diff(self._h, None, ctypes.byref(v)))
+        return v.value
+
+    def rel_diff(self, other: "State") -> float:
+        """max |a_i - b_i| / max |a_i|: the error bar that scales with the
+        amplitudes (at 30+ qubits they are ~1e-5, where an absolute 1e-12
+        could not tell FP32 from FP64 arithmetic)."""
+        return self.max_abs_diff(other) / self.max_abs()
 
     def probability(self, qubit: int, value: int = 0) -> float:
         v = ctypes.c_double()

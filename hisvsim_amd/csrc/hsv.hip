@@ -72,6 +72,9 @@ struct hsv_state {
   // by default: at 31 qubits it measured no gain (the extra layout change
   // costs what the coalescing saves).
   int io_bits = 0;
+  // hsv_permute_qubits: 0 = in place when a second state does not fit or
+  // from 32 qubits on, 1 = always in place, 2 = always out of place
+  int permute_in_place = 0;
   BfTune bf;                         // BF16-split fused kernel form and launch shape (see hsv_tune)
   // The state is |0...0> (set by create and reset, cleared by every call
   // that may change it): the chain then skips the tiles that are still zero.
@@ -578,7 +581,12 @@ static int permute_in_place(hsv_state* s, const int* new_slot, bool renorm = fal
   return end_timed(s);
 }
 
-// Out-of-place qubit permutation: amplitude bit s moves to bit new_slot[s].
+// Qubit permutation: amplitude bit s moves to bit new_slot[s]. Out of place
+// (one k_permute pass into a second buffer, which then replaces the state)
+// when a second state fits in free HBM next to a 1 GiB margin; otherwise, or
+// from 32 qubits on (a 33-qubit complex128 state is 137 GB: two of them leave
+// no room for anything else), or when permute_in_place is 1, in place as two
+// k_swap_tiles involution passes (twice the traffic, no allocation).
 static int permute_impl(hsv_state* s, const int* new_slot) {
   uint64_t seen = 0;
   bool identity = true;
@@ -589,8 +597,17 @@ static int permute_impl(hsv_state* s, const int* new_slot) {
     identity = identity && new_slot[q] == q;
   }
   if (identity) return HSV_OK;
+  const size_t bytes = s->len * s->elem();
+  bool in_place = s->permute_in_place == 1 || (s->permute_in_place == 0 && s->n >= 32);
+  if (s->permute_in_place == 0 && !in_place) {
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    in_place = free_b < bytes + (size_t(1) << 30);
+  }
+  if (in_place && s->n >= 10) return permute_in_place(s, new_slot);
+  if (s->permute_in_place == 1) return fail(HSV_ERR_LIMIT, "the in-place permutation needs at least 10 qubits");
   void* out = nullptr;
-  if (hipMalloc(&out, s->len * s->elem()) != hipSuccess)
+  if (hipMalloc(&out, bytes) != hipSuccess)
     return fail(HSV_ERR_ALLOC, "no room for the permutation's second buffer");
   int* dslot = nullptr;
   if (hipMalloc(reinterpret_cast<void**>(&dslot), sizeof(int) * 64) != hipSuccess) {
@@ -880,8 +897,12 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
     std::string err;
     if (int rc = fused_part_bf_layout(s->amp, n, s->dtype, s->stream, &s->fused_scratch, part, tile[p],
                                       layout[p].data(), layout[p + 1].data(), live, s->bf, &err)) {
-      restore(layout[p], false);
-      return fail(rc, "part " + std::to_string(p) + ": " + err);
+      // close this part's timing pair, then bring the parts applied so far
+      // back to the standard layout; a failed restore is reported too
+      std::string msg = "part " + std::to_string(p) + ": " + err;
+      if (end_timed(s)) msg += "; " + g_err;
+      if (restore(layout[p], false)) msg += "; the layout restore failed as well (" + g_err + ")";
+      return fail(rc, msg);
     }
     for (const Lowered& L : part) {
       if (L.kind == K_IDENT) continue;
@@ -952,9 +973,9 @@ int hsv_inner(hsv_state* a, hsv_state* b, double* out2) {
 }
 
 int hsv_max_abs_diff(hsv_state* a, hsv_state* b, double* out) {
-  if (!a || !b || !out) return fail(HSV_ERR_ARG, "null argument");
-  if (a->n != b->n || a->dtype != b->dtype) return fail(HSV_ERR_ARG, "states differ in size or dtype");
-  HIPCHK(hipStreamSynchronize(b->stream));
+  if (!a || !out) return fail(HSV_ERR_ARG, "null argument");
+  if (b && (a->n != b->n || a->dtype != b->dtype)) return fail(HSV_ERR_ARG, "states differ in size or dtype");
+  if (b) HIPCHK(hipStreamSynchronize(b->stream));
   return reduce<RD_MAXDIFF>(a, b, 0, out, nullptr);
 }
 
@@ -1141,6 +1162,7 @@ int hsv_tune(hsv_state* s, const char* key, int value) {
   else if (k == "part_tiles_per_wg" && value >= 0 && value <= 64) s->bf.part_tpw = value;
   else if (k == "tile_order" && value >= 0 && value <= 3) s->bf.tile_order = value;
   else if (k == "part13_u" && value >= 0 && value <= 2) s->bf.part13 = value;
+  else if (k == "permute_in_place" && value >= 0 && value <= 2) s->permute_in_place = value;
   else return fail(HSV_ERR_ARG, "unknown tuning key or value out of range: " + k);
   return HSV_OK;
 }

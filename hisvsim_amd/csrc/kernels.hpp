@@ -318,8 +318,8 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce(const typename VecT<R>::ty
         const double u = b[i].x, v = b[i].y;
         s0 += x * u + y * v;   // Re(conj(a) b)
         s1 += x * v - y * u;   // Im(conj(a) b)
-      } else {                 // RD_MAXDIFF
-        const double dx = x - (double)b[i].x, dy = y - (double)b[i].y;
+      } else {                 // RD_MAXDIFF; max |a_i| without b
+        const double dx = b ? x - (double)b[i].x : x, dy = b ? y - (double)b[i].y : y;
         s0 = fmax(s0, sqrt(dx * dx + dy * dy));
       }
     }

@@ -386,6 +386,59 @@ PYBIND11_MODULE(svsimulator_py_wrapper, m) {
       },
       py::arg("sv"), py::arg("gates"), py::arg("precision") = "f64");
 
+  // Not in the reference: the bench's default path from this module. The
+  // planner (hsv_chain_parts) cuts a gate list into parts of at most k
+  // qubits with a moving set of 5 low qubits; execute_fused_chain runs them
+  // on the BF16-split path (hsv_apply_fused_chain) and renormalises.
+  m.def(
+      "plan_chain",
+      [](const std::vector<BoundGate*>& gates, int k, int beam_width, int beam_branch, int lookahead) {
+        std::vector<uint64_t> masks(gates.size());
+        for (size_t i = 0; i < gates.size(); ++i) {
+          const hsv_gate& g = gates[i]->g;
+          uint64_t m = 0;
+          if (g.type != HSV_G_I) {   // the chain leaves identities out of a part's tile
+            for (int j = 0; j < g.num_controls; ++j) m |= 1ull << g.controls[j];
+            for (int j = 0; j < g.num_targets; ++j) m |= 1ull << g.targets[j];
+          }
+          masks[i] = m;
+        }
+        std::vector<int32_t> part(std::max<size_t>(1, gates.size()));
+        std::vector<int32_t> lows(5 * (gates.size() + 1));
+        int np = 0;
+        {
+          py::gil_scoped_release nogil;
+          check(hsv_chain_parts(masks.data(), (int)gates.size(), k, 5, lookahead, 24, beam_width, beam_branch,
+                                part.data(), lows.data(), &np),
+                "plan_chain");
+        }
+        part.resize(gates.size());
+        std::vector<std::vector<int>> lv(np);
+        for (int p = 0; p < np; ++p) lv[p].assign(lows.begin() + 5 * p, lows.begin() + 5 * p + 5);
+        return py::make_tuple(part, lv);
+      },
+      py::arg("gates"), py::arg("k") = 13, py::arg("beam_width") = 16, py::arg("beam_branch") = 8,
+      py::arg("lookahead") = 13);
+  m.def(
+      "execute_fused_chain",
+      [](StateVector& sv, const std::vector<std::vector<BoundGate*>>& parts, const std::vector<std::vector<int>>& lows,
+         bool renormalize) {
+        if (lows.size() != parts.size()) throw py::value_error("one low set per part");
+        std::vector<hsv_gate> v;
+        std::vector<int> sizes, lw;
+        for (size_t p = 0; p < parts.size(); ++p) {
+          if (lows[p].size() != 5) throw py::value_error("each low set names 5 qubits");
+          for (auto* b : parts[p]) v.push_back(b->g);
+          sizes.push_back((int)parts[p].size());
+          lw.insert(lw.end(), lows[p].begin(), lows[p].end());
+        }
+        const int mode = HSV_FUSED_BF16SPLIT | (renormalize ? HSV_CHAIN_RENORMALIZE : 0);
+        py::gil_scoped_release nogil;
+        check(hsv_apply_fused_chain(sv.h, v.data(), (int)v.size(), sizes.data(), (int)sizes.size(), lw.data(), mode),
+              "execute_fused_chain");
+      },
+      py::arg("sv"), py::arg("parts"), py::arg("lows"), py::arg("renormalize") = true);
+
   m.def("obtain_apply_time", []() { long long a; hsv_get_counters(&a, nullptr, nullptr, nullptr); return a; });
   m.def("obtain_gate_counter", []() { long long g; hsv_get_counters(nullptr, &g, nullptr, nullptr); return g; });
   m.def("obtain_move_counter", []() { return 0LL; });  // reset to 0 after every part (execute.hpp:306)

@@ -37,6 +37,14 @@ def gate_qubits(g: Gate) -> set:
     return set(g.controls) | set(g.targets)
 
 
+def tile_qubits(g: Gate) -> set:
+    """The qubits a gate needs in a BF16-split tile: none for an identity
+    (hsv_apply_fused_chain leaves I gates out of a part's tile, so a qubit
+    that only an I touches must not count towards the tile or be offered as
+    a next low slot)."""
+    return set() if g.name.upper() == "I" else gate_qubits(g)
+
+
 def nat_parts(gates: Sequence[Gate], k: int, always: Sequence[int] = ()) -> List[List[Gate]]:
     """``always`` as in ``greedy_parts``: qubits counted in every part."""
     base = set(always)
@@ -350,7 +358,7 @@ def chain_parts(gates: Sequence[Gate], k: int, low: int = 5, lookahead: int = 13
     from itertools import combinations
     if low > k:
         raise ValueError("more low qubits than the part width")
-    qsets = [frozenset(gate_qubits(g)) for g in gates]
+    qsets = [frozenset(tile_qubits(g)) for g in gates]
     nq = max((max(q) for q in qsets if q), default=-1) + 1
     if nq < low:
         raise ValueError(f"the circuit has {nq} qubits: fewer than the {low} low slots")
@@ -402,9 +410,13 @@ def merge_parts(parts: Sequence[Sequence[Gate]], k: int) -> List[List[Gate]]:
     return out
 
 
-def check_parts(gates: Sequence[Gate], parts: Sequence[Sequence[Gate]], k: int | None = None) -> None:
+def check_parts(gates: Sequence[Gate], parts: Sequence[Sequence[Gate]], k: int | None = None,
+                identities_free: bool = False) -> None:
     """Raises ValueError unless ``parts`` is a k-bounded acyclic partition of
-    ``gates`` whose sequence respects every qubit's gate order."""
+    ``gates`` whose sequence respects every qubit's gate order. With
+    ``identities_free`` (the chain planner's view) I gates occupy no qubit:
+    they neither count towards k nor order anything."""
+    qubits_of = tile_qubits if identities_free else gate_qubits
     pos = {}
     for i, g in enumerate(gates):
         pos.setdefault(g, []).append(i)
@@ -415,14 +427,14 @@ def check_parts(gates: Sequence[Gate], parts: Sequence[Sequence[Gate]], k: int |
     def per_qubit(seq):
         d: Dict[int, list] = {}
         for g in seq:
-            for q in gate_qubits(g):
+            for q in qubits_of(g):
                 d.setdefault(q, []).append(g)
         return d
     if per_qubit(gates) != per_qubit(flat):
         raise ValueError("parts reorder gates on some qubit")
     if k is not None:
         for p in parts:
-            if len(set().union(*[gate_qubits(g) for g in p])) > k:
+            if len(set().union(*[qubits_of(g) for g in p])) > k:
                 raise ValueError("a part exceeds k qubits")
 
 

@@ -129,7 +129,12 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
 
 /* Replaces the qubit exchange of SvSimMPI::gather_qubits / gather_slots
  * (svsim-mpi.hpp:123-173, mpi_redistributer.hpp:265-344) on one device:
- * amplitude bit s moves to bit new_slot[s]. */
+ * amplitude bit s moves to bit new_slot[s]. Out of place (one pass into a
+ * second state-sized buffer, which replaces the first) when that buffer fits
+ * in free HBM with 1 GiB to spare and n < 32; otherwise in place, as two
+ * tiled swap passes (the permutation as two involutions; no allocation,
+ * twice the traffic, needs n >= 10). hsv_tune "permute_in_place" forces
+ * either form. Exact in both. */
 int hsv_permute_qubits(hsv_state* s, const int* new_slot);
 
 /* Amplitude access (the reference has none beyond dump_state,
@@ -151,7 +156,8 @@ int hsv_renormalize(hsv_state* s, double* norm2_before);
 /* <a|b> as (re, im). */
 int hsv_inner(hsv_state* a, hsv_state* b, double* out2);
 /* max |a_i - b_i|; replaces the amplitude loop of operator== and state_equal
- * (state_vector.hpp:1003-1010, 1053-1072; threshold 1e-10, types.hpp:10). */
+ * (state_vector.hpp:1003-1010, 1053-1072; threshold 1e-10, types.hpp:10).
+ * b = NULL gives max |a_i| (the scale for relative error bars). */
 int hsv_max_abs_diff(hsv_state* a, hsv_state* b, double* out);
 /* Replaces StateVector::probability (state_vector.hpp:897-923). */
 int hsv_probability(hsv_state* s, int qubit, int value, double* out);
@@ -240,6 +246,9 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *                 prefetched into registers during the steps (parts of up
  *                 to 11 steps); 2 = as 0 but each wave reads U from L2
  *                 straight into registers (no LDS buffer);
+ *   "permute_in_place" hsv_permute_qubits: 0 (the default) = in place from
+ *                 32 qubits on or when a second state does not fit, 1 =
+ *                 always in place, 2 = always out of place;
  *   "tile_order"  order in which the BF16 tile kernels' workgroups take
  *                 tiles: 1..3 = the highest 1..3 slots outside the tile vary
  *                 fastest, so concurrent tiles spread over the halves

@@ -60,45 +60,83 @@ def test_qft24_gaussian_closed_form():
         assert np.abs(s.read() - want).max() <= 1e-11
 
 
-def test_qft30_paths_agree():
-    """QFT30 (435 controlled phases, 15 swaps) on a dense input: fused FP64
-    (parts of <= 7 qubits) equals per-gate FP64 to 1e-12, the BF16-split
-    fused path is within 1e-6 infidelity, every path keeps the norm."""
+def test_qft30_gaussian_three_paths():
+    """QFT30 as configured: a seeded i.i.d. complex Gaussian input (normalised),
+    H + controlled phase (the reference's controlled R1) + the reversal
+    swaps, parts of at most 10 qubits. Per-gate FP64, the part path (parts
+    of <= 10 qubits, the reference's gather_and_execute_on), fused FP64 and
+    fused FP32-split MFMA (parts of <= 7 qubits, the fused kernels' width)
+    from the same input. Bars: the FP64 paths equal per-gate to 1e-12
+    relative to the largest amplitude; FP32-split within infidelity 1e-6
+    and max error 1e-6 relative."""
+    n = 30
+    gates = qft(n)
+    x = gaussian_state(n, seed=30)
+    parts10 = closure_parts(gates, 10)
+    parts7 = closure_parts(gates, 7)
+    check_parts(gates, parts10, 10)
+    check_parts(gates, parts7, 7)
+    with capi.State(n) as a, capi.State(n) as b:
+        a.write(x)
+        a.apply_gates(gates)
+        scale = a.max_abs()
+        assert abs(a.norm2() - 1) <= TOL
+        b.write(x)
+        for p in parts10:
+            b.apply_part(p, 0)
+        assert a.max_abs_diff(b) <= TOL * scale
+        b.write(x)
+        for p in parts7:
+            b.apply_fused(p, "f64")
+        assert a.max_abs_diff(b) <= TOL * scale
+        assert abs(b.norm2() - 1) <= TOL
+        b.write(x)
+        for p in parts7:
+            b.apply_fused(p, "f32split")
+        infid = fused_infidelity(a, b)
+        rel = a.max_abs_diff(b) / scale
+        print(f"QFT30 FP32-split: infidelity {infid:.3e}, max error {rel:.3e} relative", flush=True)
+        assert infid <= 1e-6
+        assert rel <= 1e-6
+
+
+def test_qft30_bf16split_path():
+    """The same QFT30 circuit on a dense product-state input through the
+    BF16-split fused path (parts of <= 7 qubits) against per-gate FP64:
+    infidelity within 1e-6 after renormalisation, unit norm."""
     n = 30
     gates = qft(n)
     prep = [Gate("U3", (q,), (), (0.3 + 0.1 * q, 0.7 * q, 1.1 - 0.05 * q)) for q in range(n)]
     parts = closure_parts(gates, 7)
-    check_parts(gates, parts, 7)
     with capi.State(n) as a, capi.State(n) as b:
         a.apply_gates(prep + gates)
         b.apply_gates(prep)
         for p in parts:
-            b.apply_fused(p, "f64")
-        assert a.max_abs_diff(b) <= TOL
-        assert abs(b.norm2() - 1) <= TOL
-        b.reset()
-        b.apply_gates(prep)
-        for p in parts:
             b.apply_fused(p, "bf16split")
         b.renormalize()
+        assert abs(b.norm2() - 1) <= TOL
         assert fused_infidelity(a, b) <= 1e-6
 
 
 def test_rl33_depth2_part_and_fused_agree():
     """RL33 (complex128, 137 GB per state) at depth 2: the register part path
     (parts of 12 including the 5 low qubits) and the fused FP64 path equal
-    the per-gate path to 1e-12 on two full-size states."""
+    the per-gate path to 1e-12 relative to the largest amplitude on two
+    full-size states (amplitudes are ~1e-5 here: an absolute bar of 1e-12
+    would pass FP32 arithmetic)."""
     n = 33
     gates = random_layered(n, 2, seed=0)
     with capi.State(n) as a, capi.State(n) as b:
         a.apply_gates(gates)
+        scale = a.max_abs()
+        assert scale > 0
         for p in closure_parts(gates, 12, always=range(5)):
             b.apply_part(p, 0)
-        assert a.max_abs_diff(b) <= TOL
+        assert a.max_abs_diff(b) <= TOL * scale
         b.reset()
         for p in closure_parts(gates, 6):
             b.apply_fused(p, "f64")
-        assert a.max_abs_diff(b) <= TOL
+        assert a.max_abs_diff(b) <= TOL * scale
 
 
 def test_rl34_complex64_depth4():

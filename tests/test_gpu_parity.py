@@ -380,15 +380,55 @@ def test_fused_rejects_non_unitary():
             s.apply_fused([g])
 
 
+def permute_in_form(s, perm, form):
+    """hsv_permute_qubits in a forced form: 1 = in place (two involution
+    swap passes), 2 = out of place (one pass into a second buffer)."""
+    s.tune("permute_in_place", form)
+    s.permute_qubits(perm)
+
+
+@pytest.mark.parametrize("form", [1, 2])
 @pytest.mark.parametrize("seed", range(3))
-def test_permute_matches_oracle(orc, seed):
+def test_permute_matches_oracle(orc, seed, form):
     n = 13
     rng = np.random.default_rng(seed)
     perm = [int(x) for x in rng.permutation(n)]
     a = rand_state(n, seed)
     want = orc.permute(a, perm)
-    got = gpu_run(n, a, lambda s: s.permute_qubits(perm))
+    got = gpu_run(n, a, lambda s: permute_in_form(s, perm, form))
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("kind", ["cycle", "involution", "random"])
+def test_permute_in_place_matches_oracle(orc, kind, dtype):
+    """The in-place form bit for bit against the oracle at 20 qubits: one
+    long cycle (two involutions), a set of disjoint swaps (one), and a
+    random permutation; complex64 states are permuted exactly as well."""
+    n = 20
+    if kind == "cycle":
+        perm = [(q + 7) % n for q in range(n)]
+    elif kind == "involution":
+        perm = list(range(n))
+        for a_, b_ in ((0, 19), (3, 11), (4, 5), (8, 15)):
+            perm[a_], perm[b_] = b_, a_
+    else:
+        perm = [int(x) for x in np.random.default_rng(9).permutation(n)]
+    a = rand_state(n, 17)
+    if dtype == "complex64":
+        a = a.astype(np.complex64).astype(complex)
+    want = orc.permute(a, perm)
+    got = gpu_run(n, a, lambda s: permute_in_form(s, perm, 1), dtype)
+    assert np.array_equal(got, want)
+
+
+def test_permute_default_form_is_exact(orc):
+    """Whatever form the default picks (out of place below 32 qubits when a
+    second state fits), the result is the oracle's."""
+    n = 16
+    perm = [int(x) for x in np.random.default_rng(4).permutation(n)]
+    a = rand_state(n, 2)
+    assert np.array_equal(gpu_run(n, a, lambda s: s.permute_qubits(perm)), orc.permute(a, perm))
 
 
 def test_exchange_order_permutation(orc):

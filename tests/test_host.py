@@ -219,6 +219,41 @@ def test_chain_parts_native_equals_python(n, depth, k, lookahead):
     assert py[1] == nat[1]
 
 
+def identity_circuits():
+    """Circuits with I gates (QASM 'id' and 'u0' lower to I). In the first,
+    qubit 9's only gate in the first part is an I and its next real gate
+    comes first after that part: a planner that counts the I offers qubit 9
+    as a next low although the part's tile does not hold it. In the second,
+    qubit 9 sees an I before any other gate and qubit 13 only ever sees I."""
+    first = [Gate("I", (9,))]
+    for _ in range(3):
+        first += [Gate("H", (q,)) for q in range(9)]
+        first += [Gate("Z", (5 + i,), (i,)) for i in range(4)]
+    first += [Gate("Z", (10,), (9,)), Gate("Z", (11,), (10,)), Gate("H", (9,)), Gate("Z", (9,), (0,)),
+              Gate("H", (11,))]
+    second = [Gate("I", (9,)), Gate("I", (13,))] + [g for g in random_layered(14, 6, seed=11) if 13 not in g.qubits()]
+    second.insert(len(second) // 2, Gate("I", (13,)))
+    return [first, second]
+
+
+@pytest.mark.parametrize("circuit", [0, 1])
+@pytest.mark.parametrize("native,beam", [(False, (1, 1)), (True, (1, 1)), (True, (4, 2))])
+def test_chain_parts_leave_identities_out_of_tiles(circuit, native, beam):
+    """hsv_apply_fused_chain leaves I gates out of a part's tile, so the
+    planner must too: every lows[p + 1] lies within part p's non-identity
+    qubits plus lows[p], and no tile holds a qubit only an I touches."""
+    gates = identity_circuits()[circuit]
+    parts, lows = chain_parts(gates, 10, lookahead=10, native=native, beam=beam)
+    check_parts(gates, parts, 10, identities_free=True)
+    assert sum(len(p) for p in parts) == len(gates)
+    for p, (part, low) in enumerate(zip(parts, lows)):
+        tile = set(low).union(*[set(g.qubits()) for g in part if g.name != "I"])
+        assert len(tile) <= 10
+        if p + 1 < len(parts):
+            assert set(lows[p + 1]) <= tile, (p, lows[p + 1], sorted(tile))
+        assert 13 not in set(lows[p]) - set(range(5))
+
+
 @pytest.mark.parametrize("n,depth,k,beam", [(16, 12, 12, (4, 2)), (20, 10, 13, (3, 3)), (14, 8, 10, (2, 4))])
 def test_chain_parts_beam(n, depth, k, beam):
     """The beam search over low sets (hsv_chain_parts, width and branch > 1):
