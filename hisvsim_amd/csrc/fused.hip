@@ -919,12 +919,14 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     return HSV_ERR_ARG;
   }
   auto inner = [&](int q) { return __builtin_popcountll(tmask & ((1ull << ph[q]) - 1)); };
-  for (const Lowered& L : gates)
+  for (const Lowered& L : gates) {
+    if (L.kind == K_IDENT) continue;  // no-ops: their qubits need no tile slot
     for (int i = 0; i < L.nt + L.nc; ++i)
       if (!(tmask >> ph[i < L.nt ? L.t[i] : L.c[i - L.nt]] & 1)) {
         *err = "a gate acts on a qubit outside the part's tile";
         return HSV_ERR_ARG;
       }
+  }
   std::vector<uint64_t> qm;
   std::vector<const Lowered*> gl;
   for (const Lowered& L : gates) {

@@ -209,6 +209,24 @@ def test_fused_chain_matches_oracle(orc, n, depth, k, dtype):
     assert np.abs(got - want).max() <= 1e-6
 
 
+@pytest.mark.parametrize("native", [False, True])
+def test_fused_chain_with_identity_gates(orc, native):
+    """I gates (QASM 'id', 'u0') anywhere in a chain, including on a qubit
+    whose only gate in a part is the I and on a qubit that only I gates
+    touch: the planner and hsv_apply_fused_chain both leave them out of the
+    tiles, and the result matches the oracle within 1e-6."""
+    from hisvsim_amd.partition import chain_parts
+    from test_host import identity_circuits
+    for gates in identity_circuits():
+        n = 1 + max(q for g in gates for q in g.qubits())
+        parts, lows = chain_parts(gates, 10 if n < 14 else 12, lookahead=10, native=native)
+        a = rand_state(n, 5)
+        want = a.copy()
+        orc.apply_gates(want, gates)
+        got = gpu_run(n, a, lambda s: [s.apply_fused_chain(parts, lows), s.renormalize()])
+        assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
+
+
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
 @pytest.mark.parametrize("n,depth,k", [(14, 8, 12), (20, 12, 13), (22, 10, 13), (21, 9, 12)])
 def test_fused_chain_renormalize(orc, n, depth, k, dtype):
