@@ -90,6 +90,7 @@ SIGNATURES = {
     "hsv_apply_part": (ctypes.c_int, [_P, _G, ctypes.c_int, ctypes.c_int]),
     "hsv_apply_fused": (ctypes.c_int, [_P, _G, ctypes.c_int, ctypes.c_int]),
     "hsv_apply_fused_chain": (ctypes.c_int, [_P, _G, ctypes.c_int, _I, ctypes.c_int, _I, ctypes.c_int]),
+    "hsv_apply_fused_chain_noalloc": (ctypes.c_int, [_P, _G, ctypes.c_int, _I, ctypes.c_int, _I, ctypes.c_int]),
     "hsv_chain_parts": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _I]),
@@ -293,12 +294,14 @@ class State:
         _check(lib().hsv_apply_fused(self._h, arr, len(gates), m))
 
     def apply_fused_chain(self, parts: Sequence[Sequence], lows: Sequence[Sequence[int]],
-                          mode: str = "bf16split", renormalize: bool = False) -> None:
+                          mode: str = "bf16split", renormalize: bool = False, noalloc: bool = False) -> None:
         """Parts in sequence with a moving qubit layout (hsv_apply_fused_chain):
         lows[p] are the 5 qubits in slots 0-4 while part p runs (lows[0] =
         qubits 0-4); each must be a qubit of the previous part or its lows.
         renormalize: end with the state scaled to unit norm
-        (HSV_CHAIN_RENORMALIZE)."""
+        (HSV_CHAIN_RENORMALIZE). noalloc: through
+        hsv_apply_fused_chain_noalloc (no device allocation; parts of at
+        most 28 fused steps)."""
         if len(lows) != len(parts):
             raise ValueError("one lows entry per part")
         flat = [g for p in parts for g in p]
@@ -308,8 +311,9 @@ class State:
         if any(len(low) != 5 for low in lows):
             raise ValueError("lows: 5 qubits per part")
         larr = (ctypes.c_int * max(1, len(lw)))(*lw)
-        _check(lib().hsv_apply_fused_chain(self._h, arr, len(flat), sizes, len(parts), larr,
-                                           FUSED_MODES[mode] | (CHAIN_RENORMALIZE if renormalize else 0)))
+        fn = lib().hsv_apply_fused_chain_noalloc if noalloc else lib().hsv_apply_fused_chain
+        _check(fn(self._h, arr, len(flat), sizes, len(parts), larr,
+                  FUSED_MODES[mode] | (CHAIN_RENORMALIZE if renormalize else 0)))
 
     def permute_qubits(self, new_slot: Sequence[int]) -> None:
         arr = (ctypes.c_int * self.n)(*[int(x) for x in new_slot])

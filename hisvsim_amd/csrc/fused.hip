@@ -733,6 +733,11 @@ double bf16_value(uint16_t h) {
 
 int ensure_scratch(DeviceScratch* scratch, size_t bytes, hipStream_t stream, std::string* err) {
   if (scratch->bytes >= bytes) return HSV_OK;
+  if (scratch->fixed) {
+    *err = "this part needs " + std::to_string(bytes) + " bytes of device scratch; the no-allocation entry point "
+           "has " + std::to_string(scratch->bytes);
+    return HSV_ERR_LIMIT;
+  }
   if (hipStreamSynchronize(stream) != hipSuccess) { *err = "stream sync failed"; return HSV_ERR_DEVICE; }
   if (scratch->ptr) (void)hipFree(scratch->ptr);
   scratch->ptr = nullptr;

@@ -16,6 +16,9 @@ namespace hsv {
 struct DeviceScratch {
   void* ptr = nullptr;
   size_t bytes = 0;
+  // a buffer the library does not own (the no-allocation entry point's
+  // static array): never freed or grown; a larger need is an error
+  bool fixed = false;
 };
 
 // Launch choices of the BF16-split kernels.

@@ -10,6 +10,7 @@
 #include <array>
 #include <atomic>
 #include <chrono>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -44,6 +45,7 @@ int fail(int code, const std::string& msg) {
 void hsv_set_last_error(const std::string& msg) { g_err = msg; }
 
 struct hsv_state {
+  int device = 0;                  // the HIP device current at hsv_create
   int n = 0;
   int dtype = HSV_C128;
   uint64_t len = 0;
@@ -632,6 +634,43 @@ static int permute_impl(hsv_state* s, const int* new_slot) {
   return HSV_OK;
 }
 
+// Device scratch of hsv_apply_fused_chain_noalloc: one static array per GPU
+// (each device holds its own copy until the process ends). The first
+// kNoAllocPartialBytes hold the FP64 reduction partials of the
+// renormalisation (2 * kRedBlocks doubles, the size of a state's own
+// partials); the rest the fused-step tables and BF16 fragments of one part
+// at a time. Host code never passes `scratch` itself to a kernel: that name
+// is a host-side stand-in. noalloc_scratch_ptr() looks up the device address
+// with hipGetSymbolAddress, once per device with that device current.
+constexpr size_t kNoAllocScratchBytes = 256 * 1024;
+constexpr size_t kNoAllocPartialBytes = sizeof(double) * 2 * kRedBlocks;
+static_assert(kNoAllocPartialBytes % 256 == 0 && kNoAllocPartialBytes < kNoAllocScratchBytes, "scratch layout");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch_ptr(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(HSV_ERR_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return fail(HSV_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return HSV_OK;
+}
+}  // namespace
+
 // ============================================================================
 extern "C" {
 
@@ -648,6 +687,7 @@ int hsv_create(int num_qubits, int dtype, hsv_state** out) {
   s->len = 1ull << num_qubits;
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
+  s->device = dev;
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, dev));
   s->max_lds = (int)prop.sharedMemPerBlock;
@@ -914,6 +954,24 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
     g_gate_count += part_sizes[p];
   }
   return restore(layout[num_parts], renorm);
+}
+
+int hsv_apply_fused_chain_noalloc(hsv_state* s, const hsv_gate* gates, int num_gates, const int* part_sizes,
+                                  int num_parts, const int* lows, int mode) {
+  if (!s) return fail(HSV_ERR_ARG, "null handle");
+  void* p = nullptr;
+  if (int rc = noalloc_scratch_ptr(s->device, &p)) return rc;
+  // the same chain, with the static array in place of the state's own
+  // partials and fused scratch for the duration of the call
+  const DeviceScratch own = s->fused_scratch;
+  double* const own_partial = s->partial;
+  s->partial = static_cast<double*>(p);
+  s->fused_scratch = DeviceScratch{static_cast<unsigned char*>(p) + kNoAllocPartialBytes,
+                                   kNoAllocScratchBytes - kNoAllocPartialBytes, true};
+  const int rc = hsv_apply_fused_chain(s, gates, num_gates, part_sizes, num_parts, lows, mode);
+  s->fused_scratch = own;
+  s->partial = own_partial;
+  return rc;
 }
 
 int hsv_permute_qubits(hsv_state* s, const int* new_slot) {

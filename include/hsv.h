@@ -127,6 +127,21 @@ int hsv_apply_fused(hsv_state* s, const hsv_gate* gates, int num_gates, int mode
 int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, const int* part_sizes,
                           int num_parts, const int* lows, int mode);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc):
+ * hsv_apply_fused_chain with the same arguments and results, but calls no
+ * hipMalloc, hipFree or allocating library routine. Its device scratch (the
+ * renormalisation's reduction partials and one part's fused-step tables and
+ * BF16 fragments) is a static 256 KiB array per GPU, held until the process
+ * ends; a part that needs more than its 240 KiB of step data (more than 28
+ * fused 5-qubit steps) is refused with HSV_ERR_LIMIT before any pass of
+ * that part runs. Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (handles on different streams, or threads) race. Not
+ * capturable into a HIP graph (it uploads the step data from host memory and
+ * its renormalisation reads partials back to the host). */
+int hsv_apply_fused_chain_noalloc(hsv_state* s, const hsv_gate* gates, int num_gates, const int* part_sizes,
+                                  int num_parts, const int* lows, int mode);
+
 /* Replaces the qubit exchange of SvSimMPI::gather_qubits / gather_slots
  * (svsim-mpi.hpp:123-173, mpi_redistributer.hpp:265-344) on one device:
  * amplitude bit s moves to bit new_slot[s]. Out of place (one pass into a
