@@ -159,6 +159,41 @@ __global__ __launch_bounds__(4 * (1 << K)) void k_fused(typename VecT<R>::type* 
 // shared by A and B gives the same product), and result item i of lane l is
 // D[4*(l/16) + i][l%16].
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+
+// Two floats to their BF16 hi parts (round to nearest even, packed in one
+// dword as element pair 2j, 2j+1 of a bf8 operand) and the BF16 of the
+// remainders: a = hi_a + lo_a to 16 significant bits. One v_cvt_pk_bf16_f32
+// per pair for each of hi and lo; hi back to FP32 is a shift or a mask.
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t* hi, uint32_t* lo) {
+  const f2v ab = {a, b};
+  const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(ab, bf2v));
+  const f2v rest = {a - __builtin_bit_cast(float, h << 16), b - __builtin_bit_cast(float, h & 0xffff0000u)};
+  *hi = h;
+  *lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(rest, bf2v));
+}
+
+// 8 complex values (rows of a B operand) to the four BF16 operands: real hi,
+// real lo, imaginary hi, imaginary lo.
+__device__ __forceinline__ void split_bf8(const float2 (&v)[8], bf8* rh, bf8* rl, bf8* ih, bf8* il) {
+  u4v a, b, c, d;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint32_t x0, x1, y0, y1;
+    split_pair(v[2 * j].x, v[2 * j + 1].x, &x0, &x1);
+    split_pair(v[2 * j].y, v[2 * j + 1].y, &y0, &y1);
+    a[j] = x0;
+    b[j] = x1;
+    c[j] = y0;
+    d[j] = y1;
+  }
+  *rh = __builtin_bit_cast(bf8, a);
+  *rl = __builtin_bit_cast(bf8, b);
+  *ih = __builtin_bit_cast(bf8, c);
+  *il = __builtin_bit_cast(bf8, d);
+}
 
 // Outer address of tile w: the tile index deposited into the slots outside
 // the tile. top holds the highest 0..3 outer slots; they vary fastest (the
@@ -547,11 +582,28 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
       for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a[base + a_lo + ahi[wave + W * j]]);
     }
     if (it > 0) __syncthreads();  // the previous tile's stores have read ys
-    // t ^ swz[t >> 4] with swz < 32 and NTH j a multiple of 512
+    // t ^ swz[t >> 4] with swz < 32 and NTH j a multiple of 512. The swizzle
+    // is linear and t >> 4 = (x >> 4) | (NTH / 16) j with disjoint bits, so
+    // swz[t >> 4] = swz[x >> 4] ^ swz[(NTH / 16) j]: one per-thread table
+    // read, the rest wave-uniform (scalar loads of the global copy, so no
+    // per-j address registers stay live across the tile loop)
+    {
+      const uint32_t sw_x = swz[threadIdx.x >> 4];
+      // swz[(NTH / 16) 2^b], b < log2(PER), read as aligned words
+      const uint32_t* sw_words = reinterpret_cast<const uint32_t*>(tabs->swz);
+      uint32_t hb[4];
 #pragma unroll
-    for (int j = 0; j < PER; ++j)
-      ys[(threadIdx.x ^ swz[(threadIdx.x >> 4) + (NTH >> 4) * j]) + NTH * j] =
-          make_float2((float)raw[j].x, (float)raw[j].y);
+      for (int b = 0; b < 4; ++b)
+        hb[b] = (1 << b) < PER ? __builtin_amdgcn_readfirstlane(sw_words[((NTH >> 4) << b) >> 2] & 0xffu) : 0u;
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        uint32_t hj = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (j >> b & 1) hj ^= hb[b];
+        ys[(threadIdx.x ^ sw_x ^ hj) + NTH * j] = make_float2((float)raw[j].x, (float)raw[j].y);
+      }
+    }
     if (!STREAM_U && it + 1 < nt) {
       const uint64_t nbase = tile_base(t0 + it + 1, outer_mask, top);
 #pragma unroll
@@ -565,12 +617,10 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
         // registers, issued before the barrier so that their latency
         // overlaps it
         As = ufrag + s * STEP_FRAGS + lane;
-        if constexpr (CBW > 1) {
 #pragma unroll
-          for (int rb = 0; rb < 2; ++rb)
+        for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) Af[rb][q] = As[(rb * 4 + q) * 64];
-        }
+          for (int q = 0; q < 4; ++q) Af[rb][q] = As[(rb * 4 + q) * 64];
       }
       __syncthreads();  // the tile (or the previous step's results) is in LDS
       const PartBfStep& S = st[s];
@@ -585,56 +635,80 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
       } else {
         As = su + s * STEP_FRAGS + lane;
       }
-      // with several column blocks per wave, this step's U fragments are
-      // read once for all of them (the in-place writes below would otherwise
-      // force re-reads)
-      if constexpr (CBW > 1 && !U_DIRECT) {
+      // This lane reads rows g + 4e (e = 0..7) of its columns and writes
+      // result item i of row block rb to row 16 rb + 4 i + g: the same 8
+      // rows, so one set of row offsets serves both. All of the wave's column
+      // blocks are read and split before any MFMA and written after them
+      // (each wave owns its columns for the whole step), so the blocks'
+      // product chains interleave instead of running one block at a time.
+      // (two 16-bit offsets per register)
+      uint32_t roff2[4];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) Af[rb][q] = As[(rb * 4 + q) * 64];
-      }
+      for (int e = 0; e < 4; ++e) roff2[e] = uint32_t(S.rt[g + 8 * e]) | (uint32_t(S.rt[g + 8 * e + 4]) << 16);
+      auto roff = [&](int e) -> uint32_t { return e & 1 ? roff2[e >> 1] >> 16 : roff2[e >> 1] & 0xffffu; };
+      uint32_t tc[CBW];
+      bf8 xrh[CBW], xrl[CBW], xih[CBW], xil[CBW];
 #pragma unroll
       for (int w = 0; w < CBW; ++w) {
         const int col = 16 * (wave * CBW + w) + cc;
-        const int tc = S.ctl[col & 15] ^ S.cth[col >> 4];
-        bf8 xrh, xrl, xih, xil;
+        tc[w] = S.ctl[col & 15] ^ S.cth[col >> 4];
+      }
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const float2 v = ys[S.rt[g + 4 * e] ^ tc];
-          const __bf16 hr = (__bf16)v.x, hi = (__bf16)v.y;
-          xrh[e] = hr;
-          xrl[e] = (__bf16)(v.x - (float)hr);
-          xih[e] = hi;
-          xil[e] = (__bf16)(v.y - (float)hi);
+      for (int w = 0; w < CBW; ++w) {
+        float2 v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = ys[roff(e) ^ tc[w]];
+        split_bf8(v, &xrh[w], &xrl[w], &xih[w], &xil[w]);
+      }
+      // a row block's U fragments are read (once for all the wave's column
+      // blocks) just before its products; the second block's reads are
+      // issued before the first block's results are written
+      bf8 urh, url, uih, uil;
+      if constexpr (U_DIRECT) {
+        urh = Af[0][0], url = Af[0][1], uih = Af[0][2], uil = Af[0][3];
+      } else {
+        urh = As[0], url = As[64], uih = As[128], uil = As[192];
+      }
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        // Re = Ur Xr - Ui Xi, Im = Ur Xi + Ui Xr, each real product as
+        // hi*hi + hi*lo + lo*hi: four independent chains of three per block
+        v4f_t accR[CBW], accT[CBW], accI[CBW], accJ[CBW];
+#pragma unroll
+        for (int w = 0; w < CBW; ++w) {
+          const v4f_t z = {0, 0, 0, 0};
+          accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrh[w], z, 0, 0, 0);
+          accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xih[w], z, 0, 0, 0);
+          accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xih[w], z, 0, 0, 0);
+          accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrh[w], z, 0, 0, 0);
         }
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-          bf8 urh, url, uih, uil;
-          if constexpr (CBW > 1) {
-            urh = Af[rb][0], url = Af[rb][1], uih = Af[rb][2], uil = Af[rb][3];
+        for (int w = 0; w < CBW; ++w) {
+          accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrl[w], accR[w], 0, 0, 0);
+          accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xil[w], accT[w], 0, 0, 0);
+          accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xil[w], accI[w], 0, 0, 0);
+          accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrl[w], accJ[w], 0, 0, 0);
+        }
+#pragma unroll
+        for (int w = 0; w < CBW; ++w) {
+          accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xrh[w], accR[w], 0, 0, 0);
+          accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xih[w], accT[w], 0, 0, 0);
+          accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xih[w], accI[w], 0, 0, 0);
+          accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xrh[w], accJ[w], 0, 0, 0);
+        }
+        if (rb == 0) {
+          if constexpr (U_DIRECT) {
+            urh = Af[1][0], url = Af[1][1], uih = Af[1][2], uil = Af[1][3];
           } else {
-            const bf8* A = As + rb * 4 * 64;
-            urh = A[0], url = A[64], uih = A[128], uil = A[192];
+            urh = As[256], url = As[320], uih = As[384], uil = As[448];
           }
-          v4f_t accR = {0, 0, 0, 0}, accT = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
-          accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrh, accR, 0, 0, 0);
-          accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrl, accR, 0, 0, 0);
-          accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xrh, accR, 0, 0, 0);
-          accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xih, accT, 0, 0, 0);
-          accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xil, accT, 0, 0, 0);
-          accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xih, accT, 0, 0, 0);
-          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xih, accI, 0, 0, 0);
-          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xil, accI, 0, 0, 0);
-          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xih, accI, 0, 0, 0);
-          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrh, accI, 0, 0, 0);
-          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrl, accI, 0, 0, 0);
-          accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xrh, accI, 0, 0, 0);
-          // this wave read every row of its column block above: write in place
+        }
+        // every wave has read its rows of this step above: write in place
+#pragma unroll
+        for (int w = 0; w < CBW; ++w)
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            ys[S.rt[16 * rb + 4 * i + g] ^ tc] = make_float2(accR[i] - accT[i], accI[i]);
-        }
+            ys[roff(4 * rb + i) ^ tc[w]] = make_float2(accR[w][i] - accT[w][i], accI[w][i] + accJ[w][i]);
       }
     }
     __syncthreads();

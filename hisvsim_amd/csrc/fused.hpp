@@ -26,8 +26,8 @@ struct BfTune {
   int variant = 2;   // parts of 5..7 qubits: 0 / 1 column-block kernel (1: four
                      // consecutive rows per column in each load), 2 tile kernel
   int tile_tpw = 8;  // tiles of 4096 amplitudes per workgroup, tile kernel
-  int tile_order = 1;  // tile kernels: 1 = the highest slot outside the tile
-                       // varies fastest across workgroups, 0 = natural order
+  int tile_order = 4;  // tile kernels: m = the highest m slots outside the tile
+                       // vary fastest across workgroups, 0 = natural order
   int part_tpw = 0;  // tiles per workgroup, wide-part kernel (8..14 qubits);
                      // 0: 8 where the step fragments stay in LDS (the next
                      // tile is prefetched during the steps), else 1

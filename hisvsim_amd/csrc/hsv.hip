@@ -1218,7 +1218,7 @@ int hsv_tune(hsv_state* s, const char* key, int value) {
   else if (k == "fused_variant" && value >= 0 && value <= 2) s->bf.variant = value;
   else if (k == "fused_tiles_per_wg" && value >= 1 && value <= 64) s->bf.tile_tpw = value;
   else if (k == "part_tiles_per_wg" && value >= 0 && value <= 64) s->bf.part_tpw = value;
-  else if (k == "tile_order" && value >= 0 && value <= 3) s->bf.tile_order = value;
+  else if (k == "tile_order" && value >= 0 && value <= 16) s->bf.tile_order = value;
   else if (k == "part13_u" && value >= 0 && value <= 2) s->bf.part13 = value;
   else if (k == "permute_in_place" && value >= 0 && value <= 2) s->permute_in_place = value;
   else return fail(HSV_ERR_ARG, "unknown tuning key or value out of range: " + k);

@@ -265,10 +265,9 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *                 32 qubits on or when a second state does not fit, 1 =
  *                 always in place, 2 = always out of place;
  *   "tile_order"  order in which the BF16 tile kernels' workgroups take
- *                 tiles: 1..3 = the highest 1..3 slots outside the tile vary
- *                 fastest, so concurrent tiles spread over the halves
- *                 (quarters, eighths) of the state (default 1; 2 and 3
- *                 measured the same at 33 qubits); 0 = natural order. */
+ *                 tiles: m = 1..16 = the highest m slots outside the tile
+ *                 vary fastest, so concurrent tiles spread over 2^m parts
+ *                 of the state (default 4); 0 = natural order. */
 int hsv_tune(hsv_state* s, const char* key, int value);
 
 /* ---- Batched QAOA MaxCut (not in the reference) ------------------------
