@@ -272,8 +272,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=0, help="qubits (default 33; 32 for --mode permute, which "
-                                                      "needs a second state buffer)")
+    ap.add_argument("--n", type=int, default=0, help="qubits (default 33; for --mode permute the exchange "
+                                                      "runs in place from 32 qubits on, out of place below)")
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--mode", default="fusedbf16",
                     choices=["part", "fused64", "fused32", "fusedbf16", "pergate", "permute"])
@@ -301,7 +301,7 @@ def main() -> int:
               f"GPUs is a design, not built or run here.", file=sys.stderr)
         return 2
     k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 13, "pergate": 1, "permute": 3}[args.mode]
-    n, depth = args.n or (32 if args.mode == "permute" else 33), args.depth
+    n, depth = args.n or 33, args.depth
     gates = random_layered(n, depth, seed=0)
     partitioner_used, lows, plan_s = None, None, None
     if args.mode == "permute":
@@ -412,9 +412,17 @@ def main() -> int:
         # ~100 flop/B per block, under the 312 flop/B ridge of 2.5 PFLOP/s
         # over 8 TB/s, so the pass is HBM-bound too
         unit_bytes = 2 * S * 2 ** n
-        achieved = unit_bytes / (kern_ms * 1e-3) / 1e9
         kernel = {"part": "k_part_reg", "pergate": "k_gate", "permute": "k_permute",
                   "fusedbf16": "k_part_bf" if k >= 8 else "k_fused_bf"}[args.mode]
+        launches_per_call = 1
+        if args.mode == "permute" and n >= 32:
+            # in place: the permutation as two involutions, one swap-tile
+            # pass each (one when the permutation is itself an involution)
+            kernel = "k_swap_tiles"
+            involution = all(parts[parts[q]] == q for q in range(n))
+            launches_per_call = 1 if involution else 2
+            kern_ms /= launches_per_call
+        achieved = unit_bytes / (kern_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, n, args.mode, k),
                 "kernel": kernel,
@@ -467,9 +475,11 @@ def main() -> int:
                        if lows is not None else {}),
                     **({"tune": dict(tune)} if tune else {})}
                    if args.mode != "permute" else
-                   {"workload": f"exchange permutation of a {n}-qubit complex128 state ({2 ** n * 16 / 1e9:.1f} GB): "
+                   {"workload": f"exchange permutation of a {n}-qubit {args.dtype} state ({2 ** n * S / 1e9:.1f} GB): "
                                 f"qubits 9..{8 + k} to the top slots", "n": n, "mode": "permute", "k": k,
-                    "passes": passes}),
+                    "passes": passes * (launches_per_call if args.mode == "permute" else 1),
+                    "form": ("in place (k_swap_tiles involution passes)" if n >= 32 else
+                             "out of place (k_permute into a second buffer)")}),
         "hbm_gbs": 2 * S * 2 ** n / (kern_ms * 1e-3) / 1e9,
         "gate_amp_updates_per_s": total_upd / tts,
         "per_gate": per_gate,

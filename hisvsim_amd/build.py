@@ -26,7 +26,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-I", os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["hsv.hip", "fused.hip", "qaoa.hip", "mlp.hip"]
+HIP_SOURCES = ["hsv.hip", "fused.hip", "qaoa.hip", "mlp.hip", "batch.hip"]
 CXX_SOURCES = ["gates.cpp", "partition.cpp"]
 
 

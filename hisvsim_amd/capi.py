@@ -131,6 +131,14 @@ SIGNATURES = {
     "hsv_qaoa_last_ms": (ctypes.c_int, [_P, _D]),
     "hsv_qaoa_read_state": (ctypes.c_int, [_P, ctypes.c_int, _D]),
     "hsv_qaoa_total_amplitudes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "hsv_batch_create": (ctypes.c_int, [ctypes.c_int, _I, _I, _G, ctypes.c_int, ctypes.POINTER(_P)]),
+    "hsv_batch_destroy": (ctypes.c_int, [_P]),
+    "hsv_batch_set_gates": (ctypes.c_int, [_P, _G]),
+    "hsv_batch_run": (ctypes.c_int, [_P]),
+    "hsv_batch_expect_z": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64), _D]),
+    "hsv_batch_read": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "hsv_batch_tune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
+    "hsv_batch_info": (ctypes.c_int, [_P, _D, _I, ctypes.POINTER(ctypes.c_uint64)]),
     "hsv_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "hsv_kernel_times": (ctypes.c_int, [_P, _D, ctypes.c_int, _I]),
     "hsv_set_opt_slots": (None, [ctypes.c_int]),

@@ -7,6 +7,7 @@ Counterpart of the reference driver's main flow and part executor
 
     python -m hisvsim_amd.driver --app circuit.qasm [--parts --partfile F]
         [--nat K | --greedy K] [--slots] [--mode part|fused64|fused32|fusedbf16|pergate]
+        [--layout fixed|chain [--beam W,B]] [--plan file|auto]
         [--lowering reference|textbook] [--dot] [--dump] [--shots N]
         [--n T --alloc A --initn T]   (accepted for compatibility; no effect)
 
@@ -15,6 +16,13 @@ format), or from this package's natural-order (``--nat K``, the reference's
 ``*_part_nat`` method) or DAG-aware greedy partitioner (``--greedy K``).
 Without ``--parts`` every gate is applied on its own (the per-gate path).
 ``--slots`` adds the fast low qubits to every part (``set_opt_slots(1)``).
+``--mode fusedbf16 --layout chain`` runs the benchmark's path: parts of up
+to ``--nat``/``--greedy`` K (default 13) qubits from the moving-layout
+planner (``chain_parts``, beam ``--beam``) through hsv_apply_fused_chain.
+``--plan auto`` lets the learned cost model (``costmodel.choose_plan``)
+pick the mode, width and partition among its candidates; without a trained
+parameter file (or device for the network) it falls back to the plain
+partitioner (natural order, k = 10, part kernel).
 ``--dot`` writes the DAG files next to the input, ``--dump`` the
 ``<file>_exec`` ordered-gate dump. Measurements in the file are taken at
 the end (deferred): a gate on an already measured qubit is an error. One
@@ -55,6 +63,12 @@ def measured_qubits(prog: qasm.QasmProgram) -> List[int]:
 
 def plan_parts(prog: qasm.QasmProgram, args) -> List[list]:
     gates = prog.gates(args.lowering)
+    if args.layout == "chain":
+        from .partition import chain_parts
+        beam = tuple(int(x) for x in args.beam.split(","))
+        parts, lows = chain_parts(gates, args.nat or args.greedy or 13, 5, beam=beam)
+        args.lows = lows
+        return parts
     if args.partfile:
         parts = qasm.parts_from_parts_file(prog, args.partfile, args.lowering)
     elif args.nat:
@@ -65,6 +79,24 @@ def plan_parts(prog: qasm.QasmProgram, args) -> List[list]:
         parts = nat_parts(gates, 10)
     check_parts(gates, parts)
     return parts
+
+
+def auto_plan(gates, n: int):
+    """The cost model's pick among its candidate plans, or the plain
+    partitioner when no trained model is at hand."""
+    from . import costmodel
+    try:
+        model = costmodel.MLP.load()
+    except (OSError, ValueError, KeyError, capi.HsvError) as e:
+        plan = costmodel.fallback_plan(gates, n)
+        return plan, {"source": "fallback", "reason": f"no cost model ({e})", "mode": plan["mode"],
+                      "k": plan["k"], "partitioner": plan["partitioner"]}
+    try:
+        best, plans = costmodel.choose_plan(model, gates, n)
+    finally:
+        model.close()
+    return best, {"source": "cost model", "mode": best["mode"], "k": best["k"], "partitioner": best["partitioner"],
+                  "predicted_ms": best["predicted_ms"], "candidates": len(plans)}
 
 
 def run(args) -> dict:
@@ -78,8 +110,17 @@ def run(args) -> dict:
     if args.dump:
         with open(prog.filename + "_exec", "w") as f:
             f.write(qasm.exec_dump(prog))
-    parts = plan_parts(prog, args) if args.parts else []
     n = prog.num_qubits
+    args.lows = None
+    plan_info = None
+    if args.plan == "auto":
+        plan, plan_info = auto_plan(gates, n)
+        args.parts = True
+        args.mode = {"part": "part", "fused64": "fused64", "fusedbf16": "fusedbf16", "chain": "fusedbf16"}[plan["mode"]]
+        args.lows = plan.get("lows")
+        parts = plan["parts"]
+    else:
+        parts = plan_parts(prog, args) if args.parts else []
     dtype = "complex128" if args.dtype == "auto" else args.dtype
     with capi.State(n, dtype) as s:
         s.synchronize()
@@ -89,6 +130,8 @@ def run(args) -> dict:
         elif args.mode == "part":
             for p in parts:
                 s.apply_part(p, 1 if args.slots else 0)
+        elif args.lows is not None:
+            s.apply_fused_chain(parts, args.lows, "bf16split", renormalize=True)
         else:
             fm = {"fused64": "f64", "fused32": "f32split", "fusedbf16": "bf16split"}[args.mode]
             for p in parts:
@@ -104,6 +147,7 @@ def run(args) -> dict:
     return {
         "app": args.app, "qubits": n, "gates": len(gates), "parts": len(parts) if args.parts else len(gates),
         "mode": args.mode if args.parts else "pergate", "lowering": args.lowering, "dtype": dtype,
+        "layout": "chain" if args.lows is not None else "fixed", "plan": plan_info,
         "compile_us": int(t_parse * 1e6), "sim_us": int(t_sim * 1e6), "norm2": norm2,
         "measured": meas, "p1": {str(q): p for q, p in probs.items()},
         "samples": [list(map(int, x)) for x in samples],
@@ -119,6 +163,11 @@ def main(argv=None) -> int:
     ap.add_argument("--greedy", type=int, default=0, help="DAG-aware greedy parts of K qubits")
     ap.add_argument("--slots", action="store_true", help="add the fast low qubits to every part")
     ap.add_argument("--mode", default="part", choices=["part", "fused64", "fused32", "fusedbf16", "pergate"])
+    ap.add_argument("--layout", default="fixed", choices=["fixed", "chain"],
+                    help="chain: moving low-slot layout (fusedbf16 only; parts of --nat/--greedy K, default 13)")
+    ap.add_argument("--beam", default="16,8", metavar="WIDTH,BRANCH", help="chain planner beam (1,1 = greedy)")
+    ap.add_argument("--plan", default="file", choices=["file", "auto"],
+                    help="auto: the learned cost model picks mode, width and partition (fallback: nat, k = 10)")
     ap.add_argument("--lowering", default="reference", choices=["reference", "textbook"])
     ap.add_argument("--dtype", default="auto", choices=["auto", "complex128", "complex64"],
                     help="state precision (auto: complex128)")
@@ -130,6 +179,8 @@ def main(argv=None) -> int:
     ap.add_argument("--alloc", type=int, default=0, help="allocation scheme (no effect on the GPU path)")
     ap.add_argument("--initn", type=int, default=0, help="init threads (no effect on the GPU path)")
     args = ap.parse_args(argv)
+    if args.layout == "chain" and (args.mode != "fusedbf16" or not args.parts):
+        ap.error("--layout chain runs partitioned on the BF16-split path: add --parts --mode fusedbf16")
     print(json.dumps(run(args)), flush=True)
     return 0
 

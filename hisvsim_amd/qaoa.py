@@ -169,14 +169,22 @@ class QAOABatch:
         capi._check(capi.lib().hsv_qaoa_total_amplitudes(self._h, ctypes.byref(v)))
         return v.value
 
-    def bytes_per_run(self) -> int:
-        """Algorithmic HBM bytes of one run: every gate a read and a write of
-        its instance's state (8 B per complex64 amplitude)."""
+    def bytes_per_run(self, chunk_bits: int = 12) -> int:
+        """HBM bytes of one run as the schedule moves them, 8 B per complex64
+        amplitude: the initial write, per layer one read and one write of
+        the state in stage A (the ZZ layer and RX on the chunk_bits low
+        qubits) and, for instances wider than that, one more in stage B (RX
+        on the high qubits), and the read of the cost reduction."""
         tot = 0
         for i in self.instances:
-            gates = i.n + i.p * (len(i.edges) + i.n)
-            tot += gates * 2 * 8 * 2 ** i.n
+            passes = 0.5 + i.p * (1 + (i.n > chunk_bits)) + 0.5
+            tot += int(passes * 2 * 8 * 2 ** i.n)
         return tot
+
+    def gate_updates_per_run(self) -> int:
+        """Gate-amplitude updates of one run: every gate (H layer, ZZ per edge,
+        RX per qubit) counted over its instance's whole state."""
+        return sum((i.n + i.p * (len(i.edges) + i.n)) * 2 ** i.n for i in self.instances)
 
 
 CANDIDATES = [{"chunk_bits": c, "threads": t, "grid_per_cu": g}
@@ -264,9 +272,17 @@ def main(argv=None) -> int:
         mean_ms = float(np.mean(ms))
         err = max((abs(vals[i] - expectation_p1(x.n, x.edges, x.gammas[0], x.betas[0]))
                    for i, x in enumerate(insts) if x.p == 1), default=0.0)
+        shape = tuning["best"] if tuning else (load_tuning() or {}).get("best") or {}
+        nbytes = b.bytes_per_run(int(shape.get("chunk_bits", 12)))
+        gbs = nbytes / (mean_ms * 1e-3) / 1e9
         out = {"metric": "batched QAOA MaxCut <C>", "instances": len(insts), "ms_per_batch": mean_ms,
                "min_ms": float(np.min(ms)), "instances_per_s": len(insts) / (mean_ms * 1e-3),
-               "amplitudes": b.total_amplitudes(), "hbm_gbs_algorithmic": b.bytes_per_run() / (mean_ms * 1e-3) / 1e9,
+               "amplitudes": b.total_amplitudes(),
+               "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
+                            "bytes_per_batch": nbytes,
+                            "per_batch": "state bytes x passes of the schedule (init write, stage A and B read + "
+                                         "write per layer, cost read), 8 B per complex64 amplitude"},
+               "gate_amp_updates_per_s": b.gate_updates_per_run() / (mean_ms * 1e-3),
                "p1_instances": sum(1 for x in insts if x.p == 1), "p1_max_abs_err": err,
                "tuning": tuning["best"] if tuning else (load_tuning() or {}).get("best"),
                "dtype": "complex64 state, FP64 reductions", "reps": args.reps}

@@ -294,6 +294,45 @@ int hsv_qaoa_last_ms(hsv_qaoa_batch* b, double* ms);
 int hsv_qaoa_read_state(hsv_qaoa_batch* b, int instance, double* amps);
 int hsv_qaoa_total_amplitudes(hsv_qaoa_batch* b, uint64_t* out);
 
+/* ---- Batched small circuits (not in the reference) -----------------------
+ * The reference runs one circuit per call (qasm_assembler_standalone.py:
+ * 1377-1617 per gate, 1768-2030 per part); here many circuits, or parameter
+ * sets of one circuit, of uneven qubit counts run together. Instance i has
+ * num_qubits[i] (1..30) qubits and the gates gates[gate_offsets[i] ..
+ * gate_offsets[i+1]) in order (the reference's gate numbering, controls and
+ * targets as in hsv_apply_gate; at most chunk_bits qubits per gate), and
+ * starts from |0...0>. All states share one HBM buffer in `dtype`. Each
+ * instance is cut into segments whose qubits fit a 2^chunk_bits-amplitude
+ * chunk (an instance of at most chunk_bits qubits is a single segment);
+ * stage s applies every instance's s-th segment in one persistent kernel
+ * (chunks gathered into LDS, gates applied there), and the stages are
+ * captured once into a hipGraph. Bad gates are reported at creation. */
+typedef struct hsv_batch hsv_batch;
+int hsv_batch_create(int num_instances, const int* num_qubits, const int* gate_offsets, const hsv_gate* gates,
+                     int dtype, hsv_batch** out);
+int hsv_batch_destroy(hsv_batch* b);
+/* New parameters for the same gate structure (types, controls, targets,
+ * gate for gate): the matrices are rewritten in device memory and the
+ * captured graph replays unchanged. HSV_ERR_ARG if the structure differs. */
+int hsv_batch_set_gates(hsv_batch* b, const hsv_gate* gates);
+/* Runs every instance from |0...0> (one graph replay; captured on first
+ * use and after hsv_batch_tune). */
+int hsv_batch_run(hsv_batch* b);
+/* <Z_mask> per instance after a run: sum_x |a_x|^2 (-1)^popcount(x & mask),
+ * z_masks[i] for instance i (bit q = qubit q), FP64 accumulation. */
+int hsv_batch_expect_z(hsv_batch* b, const uint64_t* z_masks, double* out);
+/* The 2^num_qubits[instance] amplitudes of an instance, interleaved (re, im)
+ * doubles. */
+int hsv_batch_read(hsv_batch* b, int instance, double* amps);
+/* Shape for the autotuner: "chunk_bits" (6..12 complex128, ..13 complex64;
+ * default 12), "threads" (64/128/256/512), "grid_per_cu" (1..64).
+ * Invalidates the captured graph. */
+int hsv_batch_tune(hsv_batch* b, const char* key, int value);
+/* Device time (ms) of the last run, the number of stages, and the
+ * amplitudes the stages read and write (each segment with gates moves its
+ * instance's state once), for the traffic figure. */
+int hsv_batch_info(hsv_batch* b, double* last_ms, int* stages, uint64_t* pass_amplitudes);
+
 /* ---- Cost-model MLP (not in the reference) -------------------------------
  * in -> hidden -> hidden -> 1, ReLU, FP32, trained on the GPU (MSE, Adam) by
  * this library's forward/backward kernels. Parameters are one flat array:

@@ -1,0 +1,145 @@
+"""Batched small circuits (hsv_batch_*) against the CPU oracle: every
+instance's final state, from |0...0>, for circuits of every gate type with
+controls and uneven qubit counts; instances wider than a chunk (several
+stages); complex64 storage; new parameters replayed through the same graph;
+<Z> read-outs; and QAOA MaxCut written as gates against the dedicated QAOA
+batch. Bars: 1e-12 for complex128 amplitudes, 1e-5 for complex64."""
+import math
+
+import numpy as np
+import pytest
+
+from gatesets import random_circuit
+from hisvsim_amd import capi
+from hisvsim_amd.batch import CircuitBatch, ansatz, make_batch
+from hisvsim_amd.circuit import Gate
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_state(orc, n, gates):
+    a = np.zeros(2 ** n, complex)
+    a[0] = 1
+    orc.apply_gates(a, gates)
+    return a
+
+
+def random_instances(seed, num, n_range, ngates=(20, 80)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(num):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        g = random_circuit(rng, n, int(rng.integers(ngates[0], ngates[1] + 1)), max_controls=min(2, n - 2))
+        out.append((n, g))
+    return out
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_batch_matches_oracle(orc, seed):
+    """48 random circuits of every gate type, 0-2 controls, n = 3..14 (the
+    ones above 12 qubits in several stages at the default chunk)."""
+    insts = random_instances(seed, 48, (3, 14))
+    with CircuitBatch(insts) as b:
+        b.run()
+        assert b.info()["stages"] >= 2
+        for i, (n, g) in enumerate(insts):
+            assert np.abs(b.state(i) - oracle_state(orc, n, g)).max() <= 1e-12, i
+
+
+@pytest.mark.parametrize("chunk_bits", [6, 8])
+def test_batch_wide_instances_many_stages(orc, chunk_bits):
+    """Instances of 12-18 qubits on small chunks: many segments per instance,
+    the stages mixing instances of different widths."""
+    insts = random_instances(7 + chunk_bits, 12, (12, 18), ngates=(40, 90))
+    with CircuitBatch(insts) as b:
+        b.tune("chunk_bits", chunk_bits)
+        b.run()
+        assert b.info()["stages"] > 4
+        for i, (n, g) in enumerate(insts):
+            assert np.abs(b.state(i) - oracle_state(orc, n, g)).max() <= 1e-12, i
+
+
+def test_batch_complex64(orc):
+    insts = random_instances(3, 24, (4, 15))
+    with CircuitBatch(insts, "complex64") as b:
+        b.tune("chunk_bits", 13)
+        b.run()
+        for i, (n, g) in enumerate(insts):
+            assert np.abs(b.state(i) - oracle_state(orc, n, g)).max() <= 1e-5, i
+
+
+def test_batch_new_parameters_replay_the_graph(orc):
+    """Parameter sets: the ansatz with new angles through hsv_batch_set_gates
+    (same structure), run again, each instance against the oracle; twice,
+    so the second set replays the graph captured for the first."""
+    rng = np.random.default_rng(5)
+    shape = [(n, L) for n, L in zip(rng.integers(6, 17, 16), rng.integers(1, 5, 16))]
+    sets = [[ansatz(int(n), int(L), rng.uniform(0, 2 * math.pi, 2 * n * L)) for n, L in shape] for _ in range(3)]
+    with CircuitBatch([(int(n), g) for (n, _), g in zip(shape, sets[0])]) as b:
+        for r, gl in enumerate(sets):
+            if r:
+                b.set_gates(gl)
+            b.run()
+            for i, ((n, _), g) in enumerate(zip(shape, gl)):
+                assert np.abs(b.state(i) - oracle_state(orc, int(n), g)).max() <= 1e-12, (r, i)
+        bad = [list(g) for g in sets[0]]
+        bad[3][0] = Gate("RX", bad[3][0].targets, (), bad[3][0].params)
+        with pytest.raises(capi.HsvError):
+            b.set_gates(bad)
+
+
+def test_batch_expect_z(orc):
+    insts = random_instances(11, 20, (2, 13))
+    rng = np.random.default_rng(2)
+    masks = [int(rng.integers(0, 2 ** n)) for n, _ in insts]
+    with CircuitBatch(insts) as b:
+        b.run()
+        got = b.expect_z(masks)
+        for i, (n, g) in enumerate(insts):
+            a = b.state(i)
+            par = np.array([(-1) ** bin(x & masks[i]).count("1") for x in range(2 ** n)])
+            assert abs(got[i] - float(np.sum(np.abs(a) ** 2 * par))) <= 1e-13
+
+
+def test_batch_qaoa_as_gates_matches_the_qaoa_batch():
+    """QAOA MaxCut instances written as gates (H, the reference's RZZ(gamma),
+    RX(2 beta)) through the general batch give the dedicated QAOA batch's
+    <C> = sum_edges (1 - <Z_u Z_v>) / 2."""
+    from hisvsim_amd.qaoa import QAOABatch, make_batch as qaoa_batch
+    insts = qaoa_batch(24, seed=3, n_range=(8, 16))
+    gl = []
+    for x in insts:
+        g = [Gate("H", (q,)) for q in range(x.n)]
+        for l in range(x.p):
+            g += [Gate("RZZ", (u, v), (), (x.gammas[l],)) for u, v in x.edges]
+            g += [Gate("RX", (q,), (), (2 * x.betas[l],)) for q in range(x.n)]
+        gl.append((x.n, g))
+    with QAOABatch(insts) as q:
+        want = q.run()
+    with CircuitBatch(gl) as b:
+        b.run()
+        got = np.zeros(len(insts))
+        for i, x in enumerate(insts):
+            p = np.abs(b.state(i)) ** 2
+            idx = np.arange(2 ** x.n)
+            got[i] = sum(float(np.sum(p * (((idx >> u) ^ (idx >> v)) & 1))) for u, v in x.edges)
+    assert np.abs(got - want).max() <= 1e-4 * max(1.0, float(np.abs(want).max()))
+
+
+def test_batch_rejects_bad_input():
+    with pytest.raises(capi.HsvError):
+        CircuitBatch([(4, [Gate("H", (5,))])])            # qubit outside the instance
+    with pytest.raises(capi.HsvError):
+        CircuitBatch([(4, [Gate("SWAP", (1,))])])         # SWAP takes two targets
+    with pytest.raises(capi.HsvError):
+        CircuitBatch([(0, [])])
+
+
+def test_benchmark_batch_runs():
+    """The benchmark batch (parameter sets of the ansatz, 8-20 qubits) runs
+    and keeps every instance normalised."""
+    insts = make_batch(64, seed=1)
+    with CircuitBatch(insts) as b:
+        b.run()
+        norms = b.expect_z([0] * len(insts))
+    assert np.abs(norms - 1).max() <= 1e-12

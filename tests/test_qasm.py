@@ -207,8 +207,9 @@ def test_closed_forms_on_oracle(orc, name):
 # ---- GPU: the driver end to end ------------------------------------------------------
 def _args(**kw):
     import argparse
-    base = dict(app="", parts=False, partfile="", nat=0, greedy=0, slots=False, mode="part",
-                lowering="reference", dtype="auto", dot=False, dump=False, shots=10, seed=0, n=0, alloc=0, initn=0)
+    base = dict(app="", parts=False, partfile="", nat=0, greedy=0, slots=False, mode="part", layout="fixed",
+                beam="4,2", plan="file", lowering="reference", dtype="auto", dot=False, dump=False, shots=10,
+                seed=0, n=0, alloc=0, initn=0)
     base.update(kw)
     return argparse.Namespace(**base)
 
@@ -223,6 +224,10 @@ def _args(**kw):
     ("bv_n26", "", {"greedy": 5, "mode": "fused32"}),
     ("simple", "", {"nat": 8}),
     ("q_0_25_1", "", {"greedy": 10}),
+    ("bv_n26", "", {"mode": "fusedbf16", "layout": "chain"}),
+    ("bv_n30", "", {"mode": "fusedbf16", "layout": "chain", "nat": 12}),
+    ("bv_n26", "", {"plan": "auto"}),
+    ("q_0_25_1", "", {"plan": "auto"}),
 ])
 def test_driver_closed_form_circuits(name, partfile, extra):
     """Driver runs of the reference's circuits with their shipped parts files
@@ -232,8 +237,12 @@ def test_driver_closed_form_circuits(name, partfile, extra):
     prog = prog_of(name)
     pf = os.path.join(QDIR, partfile) if partfile else ""
     out = driver.run(_args(app=os.path.join(QDIR, name + ".qasm"), parts=True, partfile=pf, **extra))
-    # FP64 paths: 1e-12; FP32-split fused path: its 1e-6 infidelity budget
-    tol = 1e-6 if extra.get("mode") == "fused32" else 1e-12
+    # FP64 paths: 1e-12; the split-precision fused paths: their 1e-6 budget
+    tol = 1e-6 if out["mode"] in ("fused32", "fusedbf16") else 1e-12
+    if extra.get("layout") == "chain":
+        assert out["layout"] == "chain"
+    if extra.get("plan") == "auto":
+        assert out["plan"]["source"] in ("cost model", "fallback")
     assert abs(out["norm2"] - 1) <= tol
     want = closed_form(name, prog)
     probs = np.abs(want) ** 2
