@@ -215,6 +215,59 @@ def choose_plan(model: MLP, gates: Sequence[Gate], n: int) -> Tuple[dict, List[d
     return best, plans
 
 
+def default_plan(gates: Sequence[Gate], n: int) -> dict:
+    """The fixed choice without the model: the moving-layout chain at k = 13
+    (the benchmark's path, greedy planner)."""
+    return build_plan(gates, n, ("chain", 13, "chain"))
+
+
+def plan_name(plan: dict) -> str:
+    return f"{plan['mode']} k={plan['k']} {plan['partitioner']}"
+
+
+def compare_choice(model: "MLP", gates: Sequence[Gate], n: int, s: capi.State, top: int = 3) -> dict:
+    """What the model's choice is worth on one circuit, measured: the model's
+    pick and its next-ranked candidates (``top`` plans in all), the fixed
+    default (chain, k = 13) and the plain partitioner (natural order, k = 10,
+    part kernel) are each run on the dense state ``s`` and timed by their
+    kernel times. Returns the measured and predicted ms per plan, the
+    model's regret (its pick's time over the best time measured), its gains
+    over the default and over the plain partitioner, the pick's plan-level
+    prediction error, and the per-part prediction errors."""
+    best, plans = choose_plan(model, gates, n)
+    ranked = sorted(plans, key=lambda p: p["predicted_ms"])
+    named = {}
+    for p in ranked:
+        if len(named) >= top:
+            break
+        named.setdefault(plan_name(p), p)
+    dflt = default_plan(gates, n)
+    dflt["predicted_ms"] = predict_plan_ms(model, dflt, n)
+    plain = fallback_plan(gates, n)
+    plain["predicted_ms"] = predict_plan_ms(model, plain, n)
+    named.setdefault(plan_name(dflt), dflt)
+    named.setdefault(plan_name(plain), plain)
+    measured, predicted, part_err = {}, {}, []
+    for plan in named.values():
+        run_plan(s, plan, timed=False)   # code objects and scratch, so that no plan pays them in its timing
+    for name, plan in named.items():
+        times = run_plan(s, plan)
+        measured[name] = float(times.sum())
+        predicted[name] = float(plan["predicted_ms"])
+        pred = np.array([ms_of(float(t), n) for t in model.predict(plan_features(plan, n))])
+        tp = times[:len(plan["parts"])]
+        part_err.extend((np.abs(pred - tp) / tp).tolist())
+    pick = plan_name(best)
+    fastest = min(measured.values())
+    return {"pick": pick, "measured_ms": measured, "predicted_ms": predicted,
+            "regret": measured[pick] / fastest,
+            "gain_over_default": measured[plan_name(dflt)] / measured[pick],
+            "gain_over_plain": measured[plan_name(plain)] / measured[pick],
+            "plan_rel_err": abs(predicted[pick] - measured[pick]) / measured[pick],
+            "rank_of_fastest": sorted(predicted, key=predicted.get).index(min(measured, key=measured.get)),
+            "part_rel_err": part_err}
+
+
 def run_plan(s: capi.State, plan: dict, timed: bool = True) -> np.ndarray:
     """Runs a plan on the state; returns each part's kernel time (ms), and
     for a chain plan its layout restore as one more entry."""

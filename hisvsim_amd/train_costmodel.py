@@ -30,7 +30,7 @@ import numpy as np
 
 from . import capi
 from .circuit import Gate, random_layered
-from .costmodel import (MLP, PARAMS_FILE, RESTORE_MS_PER_2_30, build_plan, candidate_specs, choose_plan, fallback_plan,
+from .costmodel import (MLP, compare_choice, PARAMS_FILE, RESTORE_MS_PER_2_30, build_plan, candidate_specs, choose_plan, fallback_plan,
                         ms_of, plan_features, predict_plan_ms, run_plan, target_of)
 
 
@@ -38,13 +38,15 @@ def log(msg):
     print(msg, flush=True)
 
 
-def off_basis(s) -> None:
-    """Resets the state and takes it off |0...0> with one untimed gate, so
-    that every pass of a plan runs over the whole state (from |0...0> the
-    chain's first passes skip the tiles that are still zero, which would
-    teach the model pass times that no feature explains)."""
+def dense_start(s) -> None:
+    """Resets the state and makes it dense with one untimed layer of U3
+    gates (a product state with every amplitude nonzero). Every pass of a
+    plan then runs over the whole state (from |0...0> the chain's first
+    passes skip the tiles that are still zero), and on dense data: the
+    matrix cores draw more power on dense operands, and a mostly-zero state
+    would time the BF16-split passes up to ~25% fast."""
     s.reset()
-    s.apply_gate(Gate("H", (s.n - 1,)))
+    s.apply_gates([Gate("U3", (q,), (), (0.7 + 0.05 * q, 0.3, 1.1)) for q in range(s.n)])
 
 
 def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10), max_seconds: float = 0.0):
@@ -72,9 +74,9 @@ def collect(num: int, seed: int, n_range=(20, 30), depth_range=(4, 10), max_seco
             if s is not None:
                 s.close()
             s = capi.State(n)
+            dense_start(s)   # stays dense: every plan is unitary (BF16 plans renormalise)
         for pi in picks:
             plan = build_plan(gates, n, specs[pi])
-            off_basis(s)
             times = run_plan(s, plan)
             for f, ms in zip(plan_features(plan, n), times):
                 X.append(f)
@@ -99,34 +101,26 @@ def restore_coefficient(restore: np.ndarray) -> float:
 
 
 def evaluate(model: MLP, num: int, seed: int, n: int = 32):
+    """32-qubit random layered circuits of depth U{30..60}: for each, the
+    model's pick, its next-ranked candidates, the fixed default (the
+    moving-layout chain at k = 13) and the plain partitioner are all run
+    and timed (costmodel.compare_choice)."""
     rng = np.random.default_rng(seed)
     rows = []
-    part_err = []
     with capi.State(n) as s:
+        dense_start(s)
         for c in range(num):
             depth = int(rng.integers(30, 61))
             gates = random_layered(n, depth, seed=int(rng.integers(1 << 30)))
-            best, plans = choose_plan(model, gates, n)
-            fb = fallback_plan(gates, n)
-            fb["predicted_ms"] = predict_plan_ms(model, fb, n)
-            res = {"depth": depth, "gates": len(gates)}
-            for name, plan in (("model", best), ("fallback", fb)):
-                off_basis(s)
-                t0 = time.perf_counter()
-                times = run_plan(s, plan)
-                wall = time.perf_counter() - t0
-                pred = np.array([ms_of(float(t), n) for t in model.predict(plan_features(plan, n))])
-                tp = times[:len(plan["parts"])]
-                part_err.extend(list(np.abs(pred - tp) / tp))
-                res[name] = {"mode": plan["mode"], "k": plan["k"], "partitioner": plan["partitioner"],
-                             "parts": len(plan["parts"]), "predicted_ms": plan["predicted_ms"],
-                             "measured_kernel_ms": float(times.sum()), "wall_s": wall}
-                log(f"[eval] circuit {c} depth {depth} {name}: {res[name]}")
-            res["speedup"] = res["fallback"]["measured_kernel_ms"] / res["model"]["measured_kernel_ms"]
-            ranked = sorted(plans, key=lambda p: p["predicted_ms"])[:5]
-            res["top_predicted"] = [{"mode": p["mode"], "k": p["k"], "partitioner": p["partitioner"],
-                                     "parts": len(p["parts"]), "predicted_ms": p["predicted_ms"]} for p in ranked]
+            res = compare_choice(model, gates, n, s)
+            res.update({"depth": depth, "gates": len(gates)})
+            log(f"[eval] circuit {c} depth {depth}: pick {res['pick']} regret {res['regret']:.3f}, "
+                f"gain over default {res['gain_over_default']:.3f}, over plain {res['gain_over_plain']:.2f}, "
+                f"plan error {res['plan_rel_err']:.3f}")
             rows.append(res)
+    part_err = [e for r in rows for e in r["part_rel_err"]]
+    for r in rows:
+        del r["part_rel_err"]
     return rows, float(np.median(part_err)), float(np.mean(part_err))
 
 
@@ -178,7 +172,12 @@ def main(argv=None) -> int:
     rows, med, mean = evaluate(model, args.eval, args.seed + 1)
     report = {"samples": len(X), "circuits": args.circuits, "train_loss": loss, "val_mse_log": val_mse,
               "val_median_rel_err": val_rel, "train_seconds": train_s,
-              "eval_part_median_rel_err": med, "eval_part_mean_rel_err": mean, "eval": rows,
+              "eval_part_median_rel_err": med, "eval_part_mean_rel_err": mean,
+              "eval_regret_max": max(r["regret"] for r in rows) if rows else None,
+              "eval_gain_over_default_min": min(r["gain_over_default"] for r in rows) if rows else None,
+              "eval_gain_over_plain_min": min(r["gain_over_plain"] for r in rows) if rows else None,
+              "eval_plan_rel_err_max": max(r["plan_rel_err"] for r in rows) if rows else None,
+              "eval": rows,
               "collect_plans": meta[:50]}
     with open(os.path.join(args.out, "costmodel_report.json"), "w") as f:
         json.dump(report, f, indent=1)

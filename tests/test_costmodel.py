@@ -113,3 +113,35 @@ def test_trained_model_beats_plain_partitioner():
     model.close()
     assert t_model < 0.8 * t_fb, (best["mode"], best["k"], best["partitioner"], t_model, t_fb)
     assert infid <= (1e-12 if best["mode"] not in ("fusedbf16", "chain") else 1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("family", ["rl32", "qft30"])
+def test_cost_model_choice_measured(family):
+    """The committed model (trained on 2000 random layered circuits of
+    20-30 qubits) on circuits outside its training range: a 32-qubit random
+    layered circuit of depth 30 and a 30-qubit QFT. Its pick, its next two
+    candidates, the fixed default (chain, k = 13) and the plain partitioner
+    (natural order, k = 10) are each run on a dense state and timed. The
+    pick must be within 10% of the fastest of them (regret), its plan-level
+    prediction within 25% of its measured time, no slower than the default
+    by more than 5%, and at least 3x (QFT: 1.5x) faster than the plain
+    partitioner."""
+    from hisvsim_amd import capi
+    from hisvsim_amd.circuit import qft
+    from hisvsim_amd.costmodel import MLP, compare_choice
+    from hisvsim_amd.train_costmodel import dense_start
+    if family == "rl32":
+        n, gates, plain_gain = 32, random_layered(32, 30, seed=21), 3.0
+    else:
+        n, gates, plain_gain = 30, qft(30), 1.5
+    model = MLP.load()
+    with capi.State(n) as s:
+        dense_start(s)
+        res = compare_choice(model, gates, n, s, top=3)
+    model.close()
+    print(f"{family}: pick {res['pick']}; measured {res['measured_ms']}; predicted {res['predicted_ms']}", flush=True)
+    assert res["regret"] <= 1.10, res
+    assert res["plan_rel_err"] <= 0.25, res
+    assert res["gain_over_default"] >= 0.95, res
+    assert res["gain_over_plain"] >= plain_gain, res
