@@ -359,6 +359,11 @@ def main() -> int:
                        "ms": float(launches[:, [i for i, f in enumerate(full) if not f]].sum(axis=1).mean())}
             launches = launches[:, [i for i, f in enumerate(full) if f]]
         kern_ms = float(launches.mean())
+        if args.mode == "pergate":
+            # the dominant kernel is the dense U3 sweep over the whole state;
+            # CZ sweeps move a quarter of it and would skew a per-launch mean
+            dense = np.array([g.name == "U3" for g in gates])
+            kern_ms = float(launches[:, dense].mean())
         passes = per_step
         log(f"[bench] TTS {tts:.3f} s, {passes} passes/step, mean full pass {kern_ms:.3f} ms"
             + (f", {partial['passes']} partial passes from |0...0> in {partial['ms']:.1f} ms" if partial else ""))

@@ -36,6 +36,7 @@ def _csvs(d: str, suffix: str) -> List[str]:
 
 def short_name(name: str) -> str:
     """Kernel name without the argument list (template arguments kept)."""
+    name = name.replace("(anonymous namespace)::", "")
     depth = 0
     for i, ch in enumerate(name):
         if ch == "<":
