@@ -420,6 +420,13 @@ def main() -> int:
         kernel = {"part": "k_part_reg", "pergate": "k_gate", "permute": "k_permute",
                   "fusedbf16": "k_part_bf" if k >= 8 else "k_fused_bf"}[args.mode]
         launches_per_call = 1
+        if args.mode == "part":
+            # the tile kernel takes every part that fits one tile with slots
+            # 0-4 (12 qubits in complex128, 13 in complex64)
+            kt = 12 if args.dtype == "complex128" else 13
+            masks = [sum({1 << q for g in p for q in (*g.targets, *g.controls)}) for p in parts]
+            if all(bin(m | (2 ** LOW_SLOTS - 1)).count("1") <= kt for m in masks):
+                kernel = "k_part_tile"
         if args.mode == "permute" and n >= 32:
             # in place: the permutation as two involutions, one swap-tile
             # pass each (one when the permutation is itself an involution)

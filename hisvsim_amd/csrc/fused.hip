@@ -195,46 +195,6 @@ __device__ __forceinline__ void split_bf8(const float2 (&v)[8], bf8* rh, bf8* rl
   *il = __builtin_bit_cast(bf8, d);
 }
 
-// Outer address of tile w: the tile index deposited into the slots outside
-// the tile. top holds the highest 0..3 outer slots; they vary fastest (the
-// lowest bits of w go to them, highest slot first), so the workgroups
-// running at one time spread over the halves / quarters of the index space:
-// measured on MI355X at 33 qubits, passes whose concurrent tiles sit in one
-// half of the state run slower.
-__device__ __forceinline__ uint64_t tile_base(uint64_t w, uint64_t outer_mask, uint64_t top) {
-  if (!top) return pdep64(w, outer_mask);
-  const int nt = __popcll(top);
-  uint64_t hi = 0, t = top;
-  for (int i = 0; i < nt; ++i) {  // bit i of w -> the (i+1)-th highest slot of top
-    const uint64_t b = 1ull << (63 - __clzll(t));
-    if (w >> i & 1) hi |= b;
-    t &= ~b;
-  }
-  return pdep64(w >> nt, outer_mask & ~top) | hi;
-}
-
-// Non-temporal 2-element loads and stores of an amplitude (one dwordx4 or
-// dwordx2 access with the streaming hint): the tile kernels touch each
-// amplitude once per pass.
-template <typename R> struct Pair2 { typedef R type __attribute__((ext_vector_type(2))); };
-template <typename R>
-__device__ __forceinline__ typename VecT<R>::type ld_stream(const typename VecT<R>::type* p) {
-  using E = typename Pair2<R>::type;
-  const E v = __builtin_nontemporal_load(reinterpret_cast<const E*>(p));
-  typename VecT<R>::type o;
-  o.x = v.x;
-  o.y = v.y;
-  return o;
-}
-template <typename R>
-__device__ __forceinline__ void st_stream(typename VecT<R>::type* p, typename VecT<R>::type x) {
-  using E = typename Pair2<R>::type;
-  E v;
-  v.x = x.x;
-  v.y = x.y;
-  __builtin_nontemporal_store(v, reinterpret_cast<E*>(p));
-}
-
 struct BfArgs {
   uint64_t nblocks;     // column blocks of 16 columns
   uint64_t outer_mask;  // outer bits above the 4 column bits

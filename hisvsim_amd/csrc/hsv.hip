@@ -63,7 +63,7 @@ struct hsv_state {
   int num_cu = 256;
   // tuning knobs (hsv_tune)
   int reg_waves = 4;                 // persistent-grid factor of the register part kernel
-  int part_kernel = 1;               // 1: register-blocked part kernel for k >= 8; 0: LDS kernel only
+  int part_kernel = 2;               // 2: FP64 tile kernel for parts that fit a tile, else 1; 1: register-blocked kernel for k >= 8; 0: LDS kernel only
   int reg_bits = 0;                  // preferred register bits (0: per dtype)
   // Slots 0..low_slots-1 join every LDS-staged part when they fit: gathered
   // runs of 2^low_slots amplitudes (512 B for complex128 at 5) sustain
@@ -447,6 +447,64 @@ int launch_reg_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t par
   return fail(HSV_ERR_LIMIT, "no register-kernel shape for a " + std::to_string(k) + "-qubit part");
 }
 
+// Tile width of the FP64 tile part kernel: 64 KB of LDS per workgroup.
+template <typename R>
+constexpr int tile_part_bits() { return sizeof(R) == 8 ? 12 : 13; }
+
+// A part on the FP64 tile kernel (k_part_tile): the tile holds the part's
+// qubits padded with the lowest free slots (so slots 0-4 first) to 2^KT
+// amplitudes; the register plan is the register kernel's, over the tile's
+// bits, with KT - 9 register bits per thread.
+template <typename R>
+int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t part_mask) {
+  constexpr int KT = tile_part_bits<R>();
+  const int n = s->n;
+  uint64_t tmask = part_mask;
+  for (int b = 0; b < n && __builtin_popcountll(tmask) < KT; ++b) tmask |= 1ull << b;
+  if (__builtin_popcountll(tmask) != KT) return fail(HSV_ERR_LIMIT, "part does not fit the tile kernel");
+  int inner_of[64], slot[KT];
+  for (int b = 0, r = 0; b < 64; ++b) {
+    inner_of[b] = (tmask >> b & 1) ? r : -1;
+    if (tmask >> b & 1) slot[r++] = b;
+  }
+  RegPlan P;
+  plan_register_part(Ls, inner_of, KT, KT - 9, &P, 0);
+  TileTabs T{};
+  for (int v = 0; v < 256; ++v)
+    for (int j = 0; j < 8; ++j) {
+      if (!(v >> j & 1)) continue;
+      if (v < 64 && j < 6) T.alo[v] += 1ull << slot[j];
+      if (6 + j < KT) T.ahi[v] += 1ull << slot[6 + j];
+    }
+  // device layout: tables | segments | gate records | 4x4 matrices
+  const size_t seg_off = sizeof(TileTabs);
+  const size_t gate_off = seg_off + P.segs.size() * sizeof(RegSeg);
+  const size_t mat_off = gate_off + P.gates.size() * sizeof(RegGate);
+  const size_t bytes = mat_off + P.mat2.size() * sizeof(double);
+  if (int rc = ensure_gate_buf(s, bytes)) return rc;
+  std::vector<unsigned char> host(bytes);
+  auto* dev = static_cast<unsigned char*>(s->gate_buf);
+  std::memcpy(host.data(), &T, sizeof(T));
+  std::memcpy(host.data() + seg_off, P.segs.data(), P.segs.size() * sizeof(RegSeg));
+  RegGate* hg = reinterpret_cast<RegGate*>(host.data() + gate_off);
+  for (size_t i = 0; i < P.gates.size(); ++i) {
+    hg[i] = P.gates[i];
+    hg[i].m2 = P.m2idx[i] >= 0 ? reinterpret_cast<const double*>(dev + mat_off) + 32 * size_t(P.m2idx[i]) : nullptr;
+  }
+  if (!P.mat2.empty()) std::memcpy(host.data() + mat_off, P.mat2.data(), P.mat2.size() * sizeof(double));
+  HIPCHK(hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, s->stream));
+  const uint64_t all = n == 64 ? ~0ull : ((1ull << n) - 1);
+  const uint64_t outer = all & ~tmask;
+  const uint64_t ntiles = 1ull << (n - KT);
+  uint64_t top = 0;  // the highest tile_order outer slots vary fastest across workgroups
+  for (int i = 0; i < s->bf.tile_order && (outer & ~top); ++i) top |= 1ull << (63 - __builtin_clzll(outer & ~top));
+  hipLaunchKernelGGL((k_part_tile<R, KT>), (unsigned)ntiles, 512, 0, s->stream, amp_as<R>(s),
+                     reinterpret_cast<const TileTabs*>(dev), reinterpret_cast<const RegSeg*>(dev + seg_off),
+                     (int)P.segs.size(), reinterpret_cast<const RegGate*>(dev + gate_off), ntiles, outer, top);
+  HIPCHK(hipGetLastError());
+  return HSV_OK;
+}
+
 template <typename R, int MODE>
 int reduce_t(hsv_state* a, hsv_state* b, int qubit, double* out0, double* out1) {
   auto* pa = amp_as<R>(a);
@@ -764,20 +822,29 @@ int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_s
   }
   if (opt_slots) part |= (s->n >= 4 ? 0xFull : ((1ull << s->n) - 1));  // execute.hpp:184-199
   s->basis0 = false;
+  // part_kernel 2: every part whose qubits and slots 0-4 fit a tile takes
+  // the FP64 tile kernel (it pads the tile with the lowest free slots itself)
+  const int kt = s->dtype == HSV_C128 ? tile_part_bits<double>() : tile_part_bits<float>();
+  const uint64_t low5 = (1ull << std::min(s->n, 5)) - 1;
+  const bool tile = s->part_kernel == 2 && s->n >= kt && __builtin_popcountll(part | low5) <= kt;
   const int kmax = max_part_qubits(s);
   // Slots 0..2 make every gathered run at least 8 consecutive amplitudes; they
-  // join the part whenever the LDS budget allows (results are unchanged).
-  for (int b = 0; b < s->low_slots && b < s->n; ++b)
-    if (!(part >> b & 1) && __builtin_popcountll(part) < kmax) part |= 1ull << b;
+  // join the part whenever the LDS budget allows and, for the register
+  // kernel, up to its widest shape of 13 qubits (results are unchanged).
+  const int kpad = s->part_kernel >= 1 ? std::min(kmax, 13) : kmax;
+  for (int b = 0; !tile && b < s->low_slots && b < s->n; ++b)
+    if (!(part >> b & 1) && __builtin_popcountll(part) < kpad) part |= 1ull << b;
   if (int rc = begin_timed(s)) return rc;
   if (__builtin_popcountll(part) > kmax) {
     // Too wide for one workgroup's LDS: apply the part gate by gate.
     for (const Lowered& L : Ls)
       if (int rc = launch_gate(s, L)) return rc;
   } else if (num_gates > 0) {
-    const bool reg = s->part_kernel == 1 && __builtin_popcountll(part) >= 8;
-    int rc = reg ? (s->dtype == HSV_C128 ? launch_reg_part_t<double>(s, Ls, part) : launch_reg_part_t<float>(s, Ls, part))
-                 : (s->dtype == HSV_C128 ? launch_part_t<double>(s, Ls, part) : launch_part_t<float>(s, Ls, part));
+    const int k = __builtin_popcountll(part);
+    const bool reg = !tile && s->part_kernel >= 1 && k >= 8 && k <= 13;
+    int rc = tile  ? (s->dtype == HSV_C128 ? launch_tile_part_t<double>(s, Ls, part) : launch_tile_part_t<float>(s, Ls, part))
+             : reg ? (s->dtype == HSV_C128 ? launch_reg_part_t<double>(s, Ls, part) : launch_reg_part_t<float>(s, Ls, part))
+                   : (s->dtype == HSV_C128 ? launch_part_t<double>(s, Ls, part) : launch_part_t<float>(s, Ls, part));
     if (rc) return rc;
   }
   g_part_count++;
@@ -1211,7 +1278,7 @@ int hsv_tune(hsv_state* s, const char* key, int value) {
   if (!s || !key) return fail(HSV_ERR_ARG, "null argument");
   const std::string k = key;
   if (k == "reg_waves" && value >= 1 && value <= 64) s->reg_waves = value;
-  else if (k == "part_kernel" && (value == 0 || value == 1)) s->part_kernel = value;
+  else if (k == "part_kernel" && value >= 0 && value <= 2) s->part_kernel = value;
   else if (k == "reg_bits" && value >= 0 && value <= 5) s->reg_bits = value;
   else if (k == "low_slots" && value >= 0 && value <= 6) s->low_slots = value;
   else if (k == "io_bits" && value >= 0 && value <= 6) s->io_bits = value;

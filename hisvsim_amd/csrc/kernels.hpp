@@ -598,6 +598,46 @@ __global__ __launch_bounds__(256) void k_permute(const typename VecT<R>::type* _
   }
 }
 
+// Outer address of tile w: the tile index deposited into the slots outside
+// the tile. top holds the highest m outer slots; they vary fastest (the
+// lowest bits of w go to them, highest slot first), so the workgroups
+// running at one time spread over 2^m parts of the index space: measured on
+// MI355X at 33 qubits, passes whose concurrent tiles sit in one half of the
+// state run slower.
+__device__ __forceinline__ uint64_t tile_base(uint64_t w, uint64_t outer_mask, uint64_t top) {
+  if (!top) return pdep64(w, outer_mask);
+  const int nt = __popcll(top);
+  uint64_t hi = 0, t = top;
+  for (int i = 0; i < nt; ++i) {  // bit i of w -> the (i+1)-th highest slot of top
+    const uint64_t b = 1ull << (63 - __clzll(t));
+    if (w >> i & 1) hi |= b;
+    t &= ~b;
+  }
+  return pdep64(w >> nt, outer_mask & ~top) | hi;
+}
+
+// Non-temporal 2-element loads and stores of an amplitude (one dwordx4 or
+// dwordx2 access with the streaming hint): the tile kernels touch each
+// amplitude once per pass.
+template <typename R> struct Pair2 { typedef R type __attribute__((ext_vector_type(2))); };
+template <typename R>
+__device__ __forceinline__ typename VecT<R>::type ld_stream(const typename VecT<R>::type* p) {
+  using E = typename Pair2<R>::type;
+  const E v = __builtin_nontemporal_load(reinterpret_cast<const E*>(p));
+  typename VecT<R>::type o;
+  o.x = v.x;
+  o.y = v.y;
+  return o;
+}
+template <typename R>
+__device__ __forceinline__ void st_stream(typename VecT<R>::type* p, typename VecT<R>::type x) {
+  using E = typename Pair2<R>::type;
+  E v;
+  v.x = x.x;
+  v.y = x.y;
+  __builtin_nontemporal_store(v, reinterpret_cast<E*>(p));
+}
+
 }  // namespace hsv
 
 // ---- register-blocked part kernel ---------------------------------------------
@@ -890,6 +930,86 @@ __global__ __launch_bounds__(1 << TB) void k_part_reg(typename VecT<R>::type* __
       const uint32_t j = jt | jr[r];
       stg<NT>(&a[base + tlo[j & 63] + thi[j >> 6]], v[r]);
     }
+  }
+}
+
+// ---- FP64 tile part kernel ----------------------------------------------------
+// The part path's exact arithmetic (one FP64 sweep per gate, as the
+// reference's execute_on) on the tile layout of the BF16-split kernels: a
+// 2^KT-amplitude tile (the part's qubits, slots 0-4 and the lowest free
+// slots) is loaded with 512-byte runs per wave into LDS, the part's gates
+// run on it segment by segment in registers (each of the 512 threads holds
+// the 2^RB amplitudes of the segment's RB register bits, as in the register
+// kernel: reg_apply), and the tile is stored the same way. One workgroup
+// per tile, two per CU (64 KB of LDS each at KT = 12 in complex128), so one
+// workgroup's loads and stores overlap the other's gates.
+struct TileTabs {
+  uint64_t alo[64], ahi[256];   // address offset of tile bits 0-5 / 6-13
+};
+
+template <typename R, int KT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_part_tile(
+    typename VecT<R>::type* __restrict__ a_vec, const TileTabs* __restrict__ tabs, const RegSeg* __restrict__ segs,
+    int nseg, const RegGate* __restrict__ gates, uint64_t ntiles, uint64_t outer_mask, uint64_t top) {
+  using V = cplx<R>;
+  using G = typename VecT<R>::type;
+  constexpr int NTH = 512, W = NTH / 64, T = 1 << KT, PER = T / NTH;
+  constexpr int RB = KT - 9, NR = 1 << RB;
+  static_assert(RB >= 1 && RB <= 5, "tile shape");
+  __shared__ V ys[T];
+  __shared__ uint64_t alo[64], ahi[T >> 6];
+  // LDS bank swizzle: tile element t at t ^ (((t >> 4) ^ (t >> 8)) & 15), so
+  // that the 16-byte elements a wave reads (its lanes spanning 6 tile bits)
+  // spread over the 16 four-bank groups for most choices of register bits
+  auto sw = [](uint32_t t) { return t ^ (((t >> 4) ^ (t >> 8)) & 15u); };
+  for (int i = threadIdx.x; i < 64; i += NTH) alo[i] = tabs->alo[i];
+  for (int i = threadIdx.x; i < (T >> 6); i += NTH) ahi[i] = tabs->ahi[i];
+  __syncthreads();
+  const uint64_t t0 = blockIdx.x;
+  if (t0 >= ntiles) return;  // uniform across the workgroup
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t base = tile_base(t0, outer_mask, top) + alo[lane];
+  // tile position t = x + NTH j: its lane bits select alo, the rest ahi
+  G raw[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a_vec[base + ahi[wave + W * j]]);
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    V v;
+    v.x = raw[j].x;
+    v.y = raw[j].y;
+    ys[sw(threadIdx.x + NTH * j)] = v;
+  }
+  for (int s = 0; s < nseg; ++s) {
+    __syncthreads();  // the tile, or the previous segment's results, is in LDS
+    const RegSeg S = segs[s];
+    uint32_t jt, jr[NR];
+    reg_layout<NR>(S.regmask, T - 1, &jt, jr);
+    V v[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) v[r] = ys[sw(jt | jr[r])];
+    // the next gate's record is fetched while the current one is applied
+    RegGate cur;
+    const int g1 = S.first + S.count;
+    if (S.first < g1) cur = gates[S.first];
+    for (int gi = S.first; gi < g1; ++gi) {
+      RegGate nxtg;
+      if (gi + 1 < g1) nxtg = gates[gi + 1];
+      reg_apply<V, R, NR, RB>(v, cur, jt, jr);
+      cur = nxtg;
+    }
+    // this thread read exactly these positions: write them back in place
+#pragma unroll
+    for (int r = 0; r < NR; ++r) ys[sw(jt | jr[r])] = v[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const V v = ys[sw(threadIdx.x + NTH * j)];
+    G o;
+    o.x = v.x;
+    o.y = v.y;
+    st_stream<R>(&a_vec[base + ahi[wave + W * j]], o);
   }
 }
 

@@ -232,7 +232,11 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
                       int* num_part_qubits, double* u_out, double* unitarity_err);
 
 /* Kernel-shape knobs for the autotuner (not in the reference):
- *   "part_kernel" 1 = register-blocked part kernel for parts of >= 8 qubits,
+ *   "part_kernel" 2 (the default) = FP64 tile kernel for every part whose
+ *                 qubits and slots 0-4 fit a 2^12-amplitude tile (2^13 in
+ *                 complex64; loads and stores in 512-byte runs, gates in
+ *                 registers segment by segment), else as 1;
+ *                 1 = register-blocked part kernel for parts of >= 8 qubits,
  *                 0 = LDS kernel for every part;
  *   "reg_bits"    preferred values per thread of the register kernel, as
  *                 log2 (2..5; 0 = default for the dtype);

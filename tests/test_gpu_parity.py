@@ -600,3 +600,48 @@ def test_register_and_lds_kernels_agree():
             a.apply_part(p, 0)
             b.apply_part(p, 0)
         assert a.max_abs_diff(b) <= TOL
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("k,low", [(3, 0), (5, 5), (8, 5), (10, 5), (7, 0), (12, 0), (13, 0), (9, 3)])
+def test_tile_part_kernel(orc, dtype, k, low):
+    """part_kernel 2: parts on the FP64 tile kernel (the part's qubits with
+    slots 0-4 padded to a 12-qubit tile for complex128, 13 for complex64;
+    parts that do not fit take the register kernel); every gate type except
+    I, 0-2 controls, the low slots added or not, against the oracle."""
+    n = 17
+    rng = np.random.default_rng(2000 + 31 * k + low)
+    names = [g for g in ALL_ONE + ALL_TWO if g != "I"]
+    parts = random_parts(rng, n, 6, k, 4 * k, names=names, max_controls=2)
+    a = rand_state(n, k + low)
+    want = a.copy()
+    for p in parts:
+        orc.apply_gates(want, p)
+    with capi.State(n, dtype) as s:
+        s.tune("part_kernel", 2)
+        s.tune("low_slots", low)
+        s.write(a)
+        for p in parts:
+            s.apply_part(p, 0)
+        got = s.read()
+    scale = max(1.0, np.abs(want).max())
+    tol = (TOL if dtype == "complex128" else 1e-5) * scale
+    assert np.abs(got - want).max() <= tol
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_tile_and_register_kernels_agree_on_layered(kernel):
+    """The tile and register part kernels against the LDS kernel on a
+    random layered circuit's greedy parts (12 qubits with the low ones), at
+    22 qubits, to 1e-12 relative to the largest amplitude."""
+    from hisvsim_amd.partition import greedy_parts
+    n = 22
+    gates = random_layered(n, 8, seed=12)
+    parts = greedy_parts(gates, 12, always=range(5))
+    with capi.State(n) as a, capi.State(n) as b:
+        a.tune("part_kernel", kernel)
+        b.tune("part_kernel", 0)
+        for p in parts:
+            a.apply_part(p, 0)
+        b.apply_gates(gates)
+        assert a.max_abs_diff(b) <= TOL * b.max_abs()
