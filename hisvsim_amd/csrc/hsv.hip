@@ -63,6 +63,7 @@ struct hsv_state {
   int num_cu = 256;
   // tuning knobs (hsv_tune)
   int reg_waves = 4;                 // persistent-grid factor of the register part kernel
+  int tile_part_wgs_per_cu = 0;      // tile part kernel: persistent workgroups per CU (0: one per tile)
   int part_kernel = 2;               // 2: FP64 tile kernel for parts that fit a tile, else 1; 1: register-blocked kernel for k >= 8; 0: LDS kernel only
   int reg_bits = 0;                  // preferred register bits (0: per dtype)
   // Slots 0..low_slots-1 join every LDS-staged part when they fit: gathered
@@ -469,6 +470,55 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
   }
   RegPlan P;
   plan_register_part(Ls, inner_of, KT, KT - 9, &P, 0);
+  // fast segments (only uncontrolled one-qubit dense gates and sign flips)
+  // become rounds: the dense gates on distinct register slots, then a run of
+  // sign flips; a dense gate after a flip, or on a slot the round already
+  // has, starts the next round (so no gate moves past one it does not
+  // commute with)
+  std::vector<TileRound> rounds;
+  for (RegSeg& S : P.segs) {
+    if (__builtin_popcount(S.regmask) != KT - 9) return fail(HSV_ERR_LIMIT, "tile layout without its register bits");
+    S.fast = 1;
+    for (int i = S.first; i < S.first + S.count; ++i) {
+      const RegGate& G = P.gates[i];
+      const bool dense1 = G.kind == GK_DENSE1 && G.cmask == 0;
+      const bool sign = G.kind == GK_PHASE1 && G.m[0] == -1.0 && G.m[1] == 0.0;
+      if (!dense1 && !sign) S.fast = 0;
+    }
+    if (!S.fast) continue;
+    const int r0 = (int)rounds.size();
+    TileRound Q{{-1, -1, -1, -1, -1}, 0, 0, 0};
+    bool open = false;
+    for (int i = S.first; i < S.first + S.count; ++i) {
+      RegGate& G = P.gates[i];
+      if (G.kind == GK_DENSE1) {
+        if (Q.ns > 0 || Q.g[G.p0] >= 0) {
+          rounds.push_back(Q);
+          Q = TileRound{{-1, -1, -1, -1, -1}, 0, 0, 0};
+        }
+        Q.g[G.p0] = i;
+      } else {
+        // the thread part of the flip's bits, and the register slots r whose
+        // bits (r's bits on the set bits of regmask, lowest first) hold the rest
+        const uint32_t rm = G.cmask & S.regmask;
+        uint32_t rsel = 0;
+        for (uint32_t r = 0; r < (1u << (KT - 9)); ++r) {
+          uint32_t jr = 0, m = S.regmask;
+          for (int b = 0; m; ++b, m &= m - 1)
+            if (r >> b & 1) jr |= m & (~m + 1);
+          if ((jr & rm) == rm) rsel |= 1u << r;
+        }
+        G.t0bit = G.cmask & ~S.regmask;
+        G.t1bit = rsel;
+        if (Q.ns == 0) Q.sfirst = i;
+        ++Q.ns;
+      }
+      open = true;
+    }
+    if (open) rounds.push_back(Q);
+    S.first = r0;
+    S.count = (int)rounds.size() - r0;
+  }
   TileTabs T{};
   for (int v = 0; v < 256; ++v)
     for (int j = 0; j < 8; ++j) {
@@ -476,9 +526,10 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
       if (v < 64 && j < 6) T.alo[v] += 1ull << slot[j];
       if (6 + j < KT) T.ahi[v] += 1ull << slot[6 + j];
     }
-  // device layout: tables | segments | gate records | 4x4 matrices
+  // device layout: tables | segments | rounds | gate records | 4x4 matrices
   const size_t seg_off = sizeof(TileTabs);
-  const size_t gate_off = seg_off + P.segs.size() * sizeof(RegSeg);
+  const size_t round_off = seg_off + P.segs.size() * sizeof(RegSeg);
+  const size_t gate_off = round_off + rounds.size() * sizeof(TileRound);
   const size_t mat_off = gate_off + P.gates.size() * sizeof(RegGate);
   const size_t bytes = mat_off + P.mat2.size() * sizeof(double);
   if (int rc = ensure_gate_buf(s, bytes)) return rc;
@@ -486,6 +537,7 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
   auto* dev = static_cast<unsigned char*>(s->gate_buf);
   std::memcpy(host.data(), &T, sizeof(T));
   std::memcpy(host.data() + seg_off, P.segs.data(), P.segs.size() * sizeof(RegSeg));
+  if (!rounds.empty()) std::memcpy(host.data() + round_off, rounds.data(), rounds.size() * sizeof(TileRound));
   RegGate* hg = reinterpret_cast<RegGate*>(host.data() + gate_off);
   for (size_t i = 0; i < P.gates.size(); ++i) {
     hg[i] = P.gates[i];
@@ -498,9 +550,16 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
   const uint64_t ntiles = 1ull << (n - KT);
   uint64_t top = 0;  // the highest tile_order outer slots vary fastest across workgroups
   for (int i = 0; i < s->bf.tile_order && (outer & ~top); ++i) top |= 1ull << (63 - __builtin_clzll(outer & ~top));
-  hipLaunchKernelGGL((k_part_tile<R, KT>), (unsigned)ntiles, 512, 0, s->stream, amp_as<R>(s),
+  // tile_part_wgs_per_cu > 0: that many workgroups per CU loop over the
+  // tiles, prefetching the next one; 0: one workgroup per tile
+  const uint64_t grid = s->tile_part_wgs_per_cu > 0
+                            ? std::min<uint64_t>(ntiles, uint64_t(s->tile_part_wgs_per_cu) * s->num_cu)
+                            : ntiles;
+  auto kern = s->tile_part_wgs_per_cu > 0 ? k_part_tile<R, KT, true> : k_part_tile<R, KT, false>;
+  hipLaunchKernelGGL(kern, (unsigned)grid, 512, 0, s->stream, amp_as<R>(s),
                      reinterpret_cast<const TileTabs*>(dev), reinterpret_cast<const RegSeg*>(dev + seg_off),
-                     (int)P.segs.size(), reinterpret_cast<const RegGate*>(dev + gate_off), ntiles, outer, top);
+                     (int)P.segs.size(), reinterpret_cast<const TileRound*>(dev + round_off),
+                     reinterpret_cast<const RegGate*>(dev + gate_off), ntiles, outer, top);
   HIPCHK(hipGetLastError());
   return HSV_OK;
 }
@@ -1279,6 +1338,7 @@ int hsv_tune(hsv_state* s, const char* key, int value) {
   const std::string k = key;
   if (k == "reg_waves" && value >= 1 && value <= 64) s->reg_waves = value;
   else if (k == "part_kernel" && value >= 0 && value <= 2) s->part_kernel = value;
+  else if (k == "tile_part_wgs_per_cu" && value >= 0 && value <= 64) s->tile_part_wgs_per_cu = value;
   else if (k == "reg_bits" && value >= 0 && value <= 5) s->reg_bits = value;
   else if (k == "low_slots" && value >= 0 && value <= 6) s->low_slots = value;
   else if (k == "io_bits" && value >= 0 && value <= 6) s->io_bits = value;

@@ -657,7 +657,7 @@ struct RegSeg {
   uint32_t regmask;
   int first;   // first gate of the segment
   int count;
-  int pad;
+  int fast;    // tile kernel: first/count index rounds (TileRound), not gates
 };
 
 // 96 bytes, so the next gate's record can be prefetched into scalar
@@ -933,6 +933,94 @@ __global__ __launch_bounds__(1 << TB) void k_part_reg(typename VecT<R>::type* __
   }
 }
 
+// Sign bit flip of a real component: XOR of its top word with s (0 or the
+// sign bit), one integer op on the high half of a double.
+__device__ __forceinline__ double flip_sign(double x, uint32_t s) {
+  return __hiloint2double(__double2hiint(x) ^ (int)s, __double2loint(x));
+}
+__device__ __forceinline__ float flip_sign(float x, uint32_t s) { return __int_as_float(__float_as_int(x) ^ (int)s); }
+
+// One-qubit dense gate without controls on register slot P. Each pair's
+// products that read a0 run first, so both outputs can land in the
+// registers their inputs held (no copies where the gate loop's paths meet).
+template <typename V, typename R, int NR, int P>
+__device__ __forceinline__ void tile_dense1(V* v, const RegGate& G) {
+  constexpr int S = 1 << P;
+  const R m0 = (R)G.m[0], m1 = (R)G.m[1], m2 = (R)G.m[2], m3 = (R)G.m[3];
+  const R m4 = (R)G.m[4], m5 = (R)G.m[5], m6 = (R)G.m[6], m7 = (R)G.m[7];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r & S) continue;
+    const R a0x = v[r].x, a0y = v[r].y, a1x = v[r + S].x, a1y = v[r + S].y;
+    // every read of a0 precedes the writes of the first output, every read
+    // of a1 but the last precedes the writes of the second
+    R t1x = fma(-a0y, m5, a0x * m4), t1y = fma(a0y, m4, a0x * m5);
+    R t0x = fma(-a0y, m1, a0x * m0), t0y = fma(a0y, m0, a0x * m1);
+    t0x = fma(-a1y, m3, fma(a1x, m2, t0x));
+    t0y = fma(a1y, m2, fma(a1x, m3, t0y));
+    t1x = fma(-a1y, m7, t1x);
+    t1y = fma(a1x, m7, t1y);
+    t1x = fma(a1x, m6, t1x);
+    t1y = fma(a1y, m6, t1y);
+    v[r].x = t0x;
+    v[r].y = t0y;
+    v[r + S].x = t1x;
+    v[r + S].y = t1y;
+  }
+}
+
+// A fast segment of the tile kernel is a list of rounds. A round applies
+// uncontrolled one-qubit dense gates on distinct register slots (they
+// commute, so in slot order), then sign flips (Z, CZ, multi-controlled Z;
+// diagonal, so they commute with each other): gates [sfirst, sfirst + ns).
+// For a sign flip the host stores in t0bit the thread-bit part of its
+// control and target bits and in t1bit the register slots r whose bits hold
+// the register part; all of a round's flips make one per-lane mask over r.
+struct TileRound {
+  int g[5];   // gate on register slot P, or -1
+  int sfirst;
+  int ns;
+  int pad;
+};
+
+template <typename V, typename R, int NR, int RB>
+__device__ __forceinline__ void tile_round(V* v, const TileRound* __restrict__ q, const RegGate* __restrict__ gates,
+                                           uint32_t jt) {
+  if (q->g[0] >= 0) tile_dense1<V, R, NR, 0>(v, gates[q->g[0]]);
+  if constexpr (RB > 1) if (q->g[1] >= 0) tile_dense1<V, R, NR, 1>(v, gates[q->g[1]]);
+  if constexpr (RB > 2) if (q->g[2] >= 0) tile_dense1<V, R, NR, 2>(v, gates[q->g[2]]);
+  if constexpr (RB > 3) if (q->g[3] >= 0) tile_dense1<V, R, NR, 3>(v, gates[q->g[3]]);
+  if constexpr (RB > 4) if (q->g[4] >= 0) tile_dense1<V, R, NR, 4>(v, gates[q->g[4]]);
+  const int s0 = q->sfirst, s1 = q->sfirst + q->ns;
+  if (s0 < s1) {
+    uint32_t mask = 0;
+    for (int i = s0; i < s1; ++i) {
+      const uint32_t tm = gates[i].t0bit, rsel = gates[i].t1bit;
+      mask ^= (jt & tm) == tm ? rsel : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const uint32_t sr = (mask << (31 - r)) & 0x80000000u;
+      v[r].x = flip_sign(v[r].x, sr);
+      v[r].y = flip_sign(v[r].y, sr);
+    }
+  }
+}
+
+// pdep of the thread index into the tile bits outside regmask (RB bits):
+// a zero bit inserted at each register position, lowest first.
+template <int RB>
+__device__ __forceinline__ uint32_t tile_thread_bits(uint32_t regmask) {
+  uint32_t x = threadIdx.x, m = regmask;
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    const uint32_t lo = (m & (~m + 1)) - 1;
+    m &= m - 1;
+    x = (x & lo) | ((x & ~lo) << 1);
+  }
+  return x;
+}
+
 // ---- FP64 tile part kernel ----------------------------------------------------
 // The part path's exact arithmetic (one FP64 sweep per gate, as the
 // reference's execute_on) on the tile layout of the BF16-split kernels: a
@@ -947,10 +1035,11 @@ struct TileTabs {
   uint64_t alo[64], ahi[256];   // address offset of tile bits 0-5 / 6-13
 };
 
-template <typename R, int KT>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_part_tile(
+template <typename R, int KT, bool PIPE>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PIPE ? 2 : 4))) void k_part_tile(
     typename VecT<R>::type* __restrict__ a_vec, const TileTabs* __restrict__ tabs, const RegSeg* __restrict__ segs,
-    int nseg, const RegGate* __restrict__ gates, uint64_t ntiles, uint64_t outer_mask, uint64_t top) {
+    int nseg, const TileRound* __restrict__ rounds, const RegGate* __restrict__ gates, uint64_t ntiles,
+    uint64_t outer_mask, uint64_t top) {
   using V = cplx<R>;
   using G = typename VecT<R>::type;
   constexpr int NTH = 512, W = NTH / 64, T = 1 << KT, PER = T / NTH;
@@ -965,51 +1054,94 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
   for (int i = threadIdx.x; i < 64; i += NTH) alo[i] = tabs->alo[i];
   for (int i = threadIdx.x; i < (T >> 6); i += NTH) ahi[i] = tabs->ahi[i];
   __syncthreads();
-  const uint64_t t0 = blockIdx.x;
+  uint64_t t0 = blockIdx.x;
   if (t0 >= ntiles) return;  // uniform across the workgroup
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t base = tile_base(t0, outer_mask, top) + alo[lane];
+  uint64_t base = tile_base(t0, outer_mask, top) + alo[lane];
   // tile position t = x + NTH j: its lane bits select alo, the rest ahi
   G raw[PER];
 #pragma unroll
   for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a_vec[base + ahi[wave + W * j]]);
+  // Workgroups loop over tiles t0, t0 + gridDim.x, ...: the next tile's loads
+  // are issued into registers before this tile's gates run, so they land
+  // while the workgroup computes. Every thread moves the same tile positions
+  // between registers and LDS on the way in and out (x + NTH j), so only the
+  // segment exchanges need barriers.
+  for (;;) {
 #pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    V v;
-    v.x = raw[j].x;
-    v.y = raw[j].y;
-    ys[sw(threadIdx.x + NTH * j)] = v;
-  }
-  for (int s = 0; s < nseg; ++s) {
-    __syncthreads();  // the tile, or the previous segment's results, is in LDS
-    const RegSeg S = segs[s];
-    uint32_t jt, jr[NR];
-    reg_layout<NR>(S.regmask, T - 1, &jt, jr);
-    V v[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) v[r] = ys[sw(jt | jr[r])];
-    // the next gate's record is fetched while the current one is applied
-    RegGate cur;
-    const int g1 = S.first + S.count;
-    if (S.first < g1) cur = gates[S.first];
-    for (int gi = S.first; gi < g1; ++gi) {
-      RegGate nxtg;
-      if (gi + 1 < g1) nxtg = gates[gi + 1];
-      reg_apply<V, R, NR, RB>(v, cur, jt, jr);
-      cur = nxtg;
+    for (int j = 0; j < PER; ++j) {
+      V v;
+      v.x = raw[j].x;
+      v.y = raw[j].y;
+      ys[sw(threadIdx.x + NTH * j)] = v;
     }
-    // this thread read exactly these positions: write them back in place
+    const uint64_t t1 = t0 + gridDim.x;
+    const bool more = PIPE && t1 < ntiles;
+    const uint64_t base1 = more ? tile_base(t1, outer_mask, top) + alo[lane] : base;
+    if (more) {
 #pragma unroll
-    for (int r = 0; r < NR; ++r) ys[sw(jt | jr[r])] = v[r];
-  }
-  __syncthreads();
+      for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a_vec[base1 + ahi[wave + W * j]]);
+    }
+    for (int s = 0; s < nseg; ++s) {
+      __syncthreads();  // the tile, or the previous segment's results, is in LDS
+      const RegSeg S = segs[s];
+      // jt: this thread's tile bits, jr[r]: register slot r's (uniform). The
+      // swizzle is linear over XOR and jt, jr[r] have disjoint bits, so
+      // sw(jt | jr[r]) = sw(jt) ^ sw(jr[r]): one XOR per LDS access.
+      const uint32_t jt = tile_thread_bits<RB>(S.regmask);
+      uint32_t jr[NR];
+      {
+        uint32_t bit[RB], m = S.regmask;
 #pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const V v = ys[sw(threadIdx.x + NTH * j)];
-    G o;
-    o.x = v.x;
-    o.y = v.y;
-    st_stream<R>(&a_vec[base + ahi[wave + W * j]], o);
+        for (int i = 0; i < RB; ++i) {
+          bit[i] = m & (~m + 1);
+          m &= m - 1;
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          uint32_t x = 0;
+#pragma unroll
+          for (int i = 0; i < RB; ++i)
+            if (r >> i & 1) x |= bit[i];
+          jr[r] = x;
+        }
+      }
+      const uint32_t swt = sw(jt);
+      V v[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) v[r] = ys[swt ^ sw(jr[r])];
+      // the next gate's record is fetched while the current one is applied
+      const int g1 = S.first + S.count;
+      if (S.fast) {
+        // rounds [first, first + count): straight-line gate code joined by
+        // plain if-then branches, so each gate's results overwrite its inputs
+        for (int ri = S.first; ri < g1; ++ri) tile_round<V, R, NR, RB>(v, rounds + ri, gates, jt);
+      } else {
+        RegGate cur;
+        if (S.first < g1) cur = gates[S.first];
+        for (int gi = S.first; gi < g1; ++gi) {
+          RegGate nxtg;
+          if (gi + 1 < g1) nxtg = gates[gi + 1];
+          reg_apply<V, R, NR, RB>(v, cur, jt, jr);
+          cur = nxtg;
+        }
+      }
+      // this thread read exactly these positions: write them back in place
+#pragma unroll
+      for (int r = 0; r < NR; ++r) ys[swt ^ sw(jr[r])] = v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const V v = ys[sw(threadIdx.x + NTH * j)];
+      G o;
+      o.x = v.x;
+      o.y = v.y;
+      st_stream<R>(&a_vec[base + ahi[wave + W * j]], o);
+    }
+    if (!more) break;
+    t0 = t1;
+    base = base1;
   }
 }
 

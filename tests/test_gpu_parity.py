@@ -629,6 +629,44 @@ def test_tile_part_kernel(orc, dtype, k, low):
     assert np.abs(got - want).max() <= tol
 
 
+@pytest.mark.parametrize("wgs", [0, 1, 3])
+def test_tile_part_kernel_layered_fast_segments(orc, wgs):
+    """Random layered circuits (U3 on every qubit, CZ on random pairs, an
+    extra Z, CCZ and controlled U3 now and then) on the tile kernel: the
+    segments of uncontrolled one-qubit gates and sign flips take the tile
+    kernel's short gate loop, the others the general one; one workgroup per
+    tile (wgs 0) or that many persistent workgroups per CU, each loading the
+    next tile while it computes. Against the oracle at 1e-12."""
+    n = 16
+    rng = np.random.default_rng(77 + wgs)
+    gates = random_layered(n, 6, seed=5 + wgs)
+    extra = []
+    for g in gates:
+        extra.append(g)
+        u = rng.random()
+        if u < 0.05:
+            extra.append(Gate("Z", (int(rng.integers(n)),)))
+        elif u < 0.08:
+            q = [int(x) for x in rng.choice(n, 3, replace=False)]
+            extra.append(Gate("Z", (q[0],), (q[1], q[2])))
+        elif u < 0.1:
+            q = [int(x) for x in rng.choice(n, 2, replace=False)]
+            extra.append(Gate("U3", (q[0],), (q[1],), (0.3, 0.7, 1.1)))
+    from hisvsim_amd.partition import closure_parts
+    parts = closure_parts(extra, 12, always=range(5))
+    a = rand_state(n, 3)
+    want = a.copy()
+    orc.apply_gates(want, extra)
+    with capi.State(n) as s:
+        s.tune("part_kernel", 2)
+        s.tune("tile_part_wgs_per_cu", wgs)
+        s.write(a)
+        for p in parts:
+            s.apply_part(p, 0)
+        got = s.read()
+    assert np.abs(got - want).max() <= TOL * max(1.0, np.abs(want).max())
+
+
 @pytest.mark.parametrize("kernel", [1, 2])
 def test_tile_and_register_kernels_agree_on_layered(kernel):
     """The tile and register part kernels against the LDS kernel on a
