@@ -63,7 +63,6 @@ struct hsv_state {
   int num_cu = 256;
   // tuning knobs (hsv_tune)
   int reg_waves = 4;                 // persistent-grid factor of the register part kernel
-  int tile_part_wgs_per_cu = 0;      // tile part kernel: persistent workgroups per CU (0: one per tile)
   int part_kernel = 2;               // 2: FP64 tile kernel for parts that fit a tile, else 1; 1: register-blocked kernel for k >= 8; 0: LDS kernel only
   int reg_bits = 0;                  // preferred register bits (0: per dtype)
   // Slots 0..low_slots-1 join every LDS-staged part when they fit: gathered
@@ -550,13 +549,7 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
   const uint64_t ntiles = 1ull << (n - KT);
   uint64_t top = 0;  // the highest tile_order outer slots vary fastest across workgroups
   for (int i = 0; i < s->bf.tile_order && (outer & ~top); ++i) top |= 1ull << (63 - __builtin_clzll(outer & ~top));
-  // tile_part_wgs_per_cu > 0: that many workgroups per CU loop over the
-  // tiles, prefetching the next one; 0: one workgroup per tile
-  const uint64_t grid = s->tile_part_wgs_per_cu > 0
-                            ? std::min<uint64_t>(ntiles, uint64_t(s->tile_part_wgs_per_cu) * s->num_cu)
-                            : ntiles;
-  auto kern = s->tile_part_wgs_per_cu > 0 ? k_part_tile<R, KT, true> : k_part_tile<R, KT, false>;
-  hipLaunchKernelGGL(kern, (unsigned)grid, 512, 0, s->stream, amp_as<R>(s),
+  hipLaunchKernelGGL((k_part_tile<R, KT>), (unsigned)ntiles, 512, 0, s->stream, amp_as<R>(s),
                      reinterpret_cast<const TileTabs*>(dev), reinterpret_cast<const RegSeg*>(dev + seg_off),
                      (int)P.segs.size(), reinterpret_cast<const TileRound*>(dev + round_off),
                      reinterpret_cast<const RegGate*>(dev + gate_off), ntiles, outer, top);
@@ -1338,7 +1331,6 @@ int hsv_tune(hsv_state* s, const char* key, int value) {
   const std::string k = key;
   if (k == "reg_waves" && value >= 1 && value <= 64) s->reg_waves = value;
   else if (k == "part_kernel" && value >= 0 && value <= 2) s->part_kernel = value;
-  else if (k == "tile_part_wgs_per_cu" && value >= 0 && value <= 64) s->tile_part_wgs_per_cu = value;
   else if (k == "reg_bits" && value >= 0 && value <= 5) s->reg_bits = value;
   else if (k == "low_slots" && value >= 0 && value <= 6) s->low_slots = value;
   else if (k == "io_bits" && value >= 0 && value <= 6) s->io_bits = value;

@@ -1035,8 +1035,8 @@ struct TileTabs {
   uint64_t alo[64], ahi[256];   // address offset of tile bits 0-5 / 6-13
 };
 
-template <typename R, int KT, bool PIPE>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PIPE ? 2 : 4))) void k_part_tile(
+template <typename R, int KT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_part_tile(
     typename VecT<R>::type* __restrict__ a_vec, const TileTabs* __restrict__ tabs, const RegSeg* __restrict__ segs,
     int nseg, const TileRound* __restrict__ rounds, const RegGate* __restrict__ gates, uint64_t ntiles,
     uint64_t outer_mask, uint64_t top) {
@@ -1054,20 +1054,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PIPE ? 2 : 
   for (int i = threadIdx.x; i < 64; i += NTH) alo[i] = tabs->alo[i];
   for (int i = threadIdx.x; i < (T >> 6); i += NTH) ahi[i] = tabs->ahi[i];
   __syncthreads();
-  uint64_t t0 = blockIdx.x;
+  const uint64_t t0 = blockIdx.x;
   if (t0 >= ntiles) return;  // uniform across the workgroup
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint64_t base = tile_base(t0, outer_mask, top) + alo[lane];
-  // tile position t = x + NTH j: its lane bits select alo, the rest ahi
-  G raw[PER];
+  const uint64_t base = tile_base(t0, outer_mask, top) + alo[lane];
+  {
+    // tile position t = x + NTH j: its lane bits select alo, the rest ahi
+    G raw[PER];
 #pragma unroll
-  for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a_vec[base + ahi[wave + W * j]]);
-  // Workgroups loop over tiles t0, t0 + gridDim.x, ...: the next tile's loads
-  // are issued into registers before this tile's gates run, so they land
-  // while the workgroup computes. Every thread moves the same tile positions
-  // between registers and LDS on the way in and out (x + NTH j), so only the
-  // segment exchanges need barriers.
-  for (;;) {
+    for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a_vec[base + ahi[wave + W * j]]);
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       V v;
@@ -1075,73 +1070,63 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(PIPE ? 2 : 
       v.y = raw[j].y;
       ys[sw(threadIdx.x + NTH * j)] = v;
     }
-    const uint64_t t1 = t0 + gridDim.x;
-    const bool more = PIPE && t1 < ntiles;
-    const uint64_t base1 = more ? tile_base(t1, outer_mask, top) + alo[lane] : base;
-    if (more) {
+  }
+  for (int s = 0; s < nseg; ++s) {
+    __syncthreads();  // the tile, or the previous segment's results, is in LDS
+    const RegSeg S = segs[s];
+    // jt: this thread's tile bits, jr[r]: register slot r's (uniform). The
+    // swizzle is linear over XOR and jt, jr[r] have disjoint bits, so
+    // sw(jt | jr[r]) = sw(jt) ^ sw(jr[r]): one XOR per LDS access.
+    const uint32_t jt = tile_thread_bits<RB>(S.regmask);
+    uint32_t jr[NR];
+    {
+      uint32_t bit[RB], m = S.regmask;
 #pragma unroll
-      for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a_vec[base1 + ahi[wave + W * j]]);
-    }
-    for (int s = 0; s < nseg; ++s) {
-      __syncthreads();  // the tile, or the previous segment's results, is in LDS
-      const RegSeg S = segs[s];
-      // jt: this thread's tile bits, jr[r]: register slot r's (uniform). The
-      // swizzle is linear over XOR and jt, jr[r] have disjoint bits, so
-      // sw(jt | jr[r]) = sw(jt) ^ sw(jr[r]): one XOR per LDS access.
-      const uint32_t jt = tile_thread_bits<RB>(S.regmask);
-      uint32_t jr[NR];
-      {
-        uint32_t bit[RB], m = S.regmask;
-#pragma unroll
-        for (int i = 0; i < RB; ++i) {
-          bit[i] = m & (~m + 1);
-          m &= m - 1;
-        }
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-          uint32_t x = 0;
-#pragma unroll
-          for (int i = 0; i < RB; ++i)
-            if (r >> i & 1) x |= bit[i];
-          jr[r] = x;
-        }
+      for (int i = 0; i < RB; ++i) {
+        bit[i] = m & (~m + 1);
+        m &= m - 1;
       }
-      const uint32_t swt = sw(jt);
-      V v[NR];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) v[r] = ys[swt ^ sw(jr[r])];
+      for (int r = 0; r < NR; ++r) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+          if (r >> i & 1) x |= bit[i];
+        jr[r] = x;
+      }
+    }
+    const uint32_t swt = sw(jt);
+    V v[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) v[r] = ys[swt ^ sw(jr[r])];
+    const int g1 = S.first + S.count;
+    if (S.fast) {
+      // rounds [first, first + count): straight-line gate code joined by
+      // plain if-then branches, so each gate's results overwrite its inputs
+      for (int ri = S.first; ri < g1; ++ri) tile_round<V, R, NR, RB>(v, rounds + ri, gates, jt);
+    } else {
       // the next gate's record is fetched while the current one is applied
-      const int g1 = S.first + S.count;
-      if (S.fast) {
-        // rounds [first, first + count): straight-line gate code joined by
-        // plain if-then branches, so each gate's results overwrite its inputs
-        for (int ri = S.first; ri < g1; ++ri) tile_round<V, R, NR, RB>(v, rounds + ri, gates, jt);
-      } else {
-        RegGate cur;
-        if (S.first < g1) cur = gates[S.first];
-        for (int gi = S.first; gi < g1; ++gi) {
-          RegGate nxtg;
-          if (gi + 1 < g1) nxtg = gates[gi + 1];
-          reg_apply<V, R, NR, RB>(v, cur, jt, jr);
-          cur = nxtg;
-        }
+      RegGate cur;
+      if (S.first < g1) cur = gates[S.first];
+      for (int gi = S.first; gi < g1; ++gi) {
+        RegGate nxtg;
+        if (gi + 1 < g1) nxtg = gates[gi + 1];
+        reg_apply<V, R, NR, RB>(v, cur, jt, jr);
+        cur = nxtg;
       }
-      // this thread read exactly these positions: write them back in place
-#pragma unroll
-      for (int r = 0; r < NR; ++r) ys[swt ^ sw(jr[r])] = v[r];
     }
-    __syncthreads();
+    // this thread read exactly these positions: write them back in place
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      const V v = ys[sw(threadIdx.x + NTH * j)];
-      G o;
-      o.x = v.x;
-      o.y = v.y;
-      st_stream<R>(&a_vec[base + ahi[wave + W * j]], o);
-    }
-    if (!more) break;
-    t0 = t1;
-    base = base1;
+    for (int r = 0; r < NR; ++r) ys[swt ^ sw(jr[r])] = v[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const V v = ys[sw(threadIdx.x + NTH * j)];
+    G o;
+    o.x = v.x;
+    o.y = v.y;
+    st_stream<R>(&a_vec[base + ahi[wave + W * j]], o);
   }
 }
 

@@ -629,17 +629,16 @@ def test_tile_part_kernel(orc, dtype, k, low):
     assert np.abs(got - want).max() <= tol
 
 
-@pytest.mark.parametrize("wgs", [0, 1, 3])
-def test_tile_part_kernel_layered_fast_segments(orc, wgs):
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_tile_part_kernel_layered_fast_segments(orc, seed):
     """Random layered circuits (U3 on every qubit, CZ on random pairs, an
     extra Z, CCZ and controlled U3 now and then) on the tile kernel: the
-    segments of uncontrolled one-qubit gates and sign flips take the tile
-    kernel's short gate loop, the others the general one; one workgroup per
-    tile (wgs 0) or that many persistent workgroups per CU, each loading the
-    next tile while it computes. Against the oracle at 1e-12."""
+    segments of uncontrolled one-qubit gates and sign flips run as rounds
+    (dense gates by register slot, then one merged sign mask), the others
+    through the general gate loop. Against the oracle at 1e-12."""
     n = 16
-    rng = np.random.default_rng(77 + wgs)
-    gates = random_layered(n, 6, seed=5 + wgs)
+    rng = np.random.default_rng(77 + seed)
+    gates = random_layered(n, 6, seed=5 + seed)
     extra = []
     for g in gates:
         extra.append(g)
@@ -659,7 +658,6 @@ def test_tile_part_kernel_layered_fast_segments(orc, wgs):
     orc.apply_gates(want, extra)
     with capi.State(n) as s:
         s.tune("part_kernel", 2)
-        s.tune("tile_part_wgs_per_cu", wgs)
         s.write(a)
         for p in parts:
             s.apply_part(p, 0)
