@@ -8,7 +8,8 @@ Counterpart of the reference driver's main flow and part executor
     python -m hisvsim_amd.driver --app circuit.qasm [--parts --partfile F]
         [--nat K | --greedy K] [--slots] [--mode part|fused64|fused32|fusedbf16|pergate]
         [--layout fixed|chain [--beam W,B]] [--plan file|auto]
-        [--lowering reference|textbook] [--dot] [--dump] [--shots N]
+        [--lowering reference|textbook] [--verify auto|none|fp64|inverse]
+        [--dot] [--dump] [--shots N]
         [--n T --alloc A --initn T]   (accepted for compatibility; no effect)
 
 Parts come from a parts file (``<order> <node> <part>`` lines, the reference
@@ -23,7 +24,13 @@ planner (``chain_parts``, beam ``--beam``) through hsv_apply_fused_chain.
 pick the mode, width and partition among its candidates; without a trained
 parameter file (or device for the network) it falls back to the plain
 partitioner (natural order, k = 10, part kernel).
-``--dot`` writes the DAG files next to the input, ``--dump`` the
+``--verify`` reports the run's accuracy (the reference's ``verify_compute``,
+:716-1198): ``fp64`` runs the circuit again on the FP64 per-gate path in a
+second state and reports the infidelity and max |a - b|; ``inverse`` applies
+the inverse circuit to the result on the FP64 per-gate path and checks that
+it returns to |0...0>; ``auto`` (the default) does ``fp64`` for the
+split-precision modes (FP32-split, BF16-split) and nothing for the FP64
+ones. ``--dot`` writes the DAG files next to the input, ``--dump`` the
 ``<file>_exec`` ordered-gate dump. Measurements in the file are taken at
 the end (deferred): a gate on an already measured qubit is an error. One
 JSON line reports the timings (µs, like the reference's counters) and, per
@@ -39,7 +46,7 @@ from typing import List
 
 import numpy as np
 
-from . import capi, qasm
+from . import capi, qasm, verify
 from .partition import check_parts, greedy_parts, nat_parts
 
 NUM_MEASURE = 10   # shots per measurement in the reference driver (qasm_assembler_standalone.py:29)
@@ -144,13 +151,22 @@ def run(args) -> dict:
         samples = (s.measure(meas, np.random.default_rng(args.seed).random(args.shots))
                    if meas and args.shots else [])
         norm2 = s.norm2()
+        mode = args.mode if args.parts else "pergate"
+        check = getattr(args, "verify", "auto")
+        if check == "auto":
+            check = "fp64" if mode in ("fused32", "fusedbf16") else "none"
+        accuracy = None
+        if check == "fp64":
+            accuracy = verify.against_fp64(s, gates)
+        elif check == "inverse":
+            accuracy = verify.by_inverse(s, gates)
     return {
         "app": args.app, "qubits": n, "gates": len(gates), "parts": len(parts) if args.parts else len(gates),
         "mode": args.mode if args.parts else "pergate", "lowering": args.lowering, "dtype": dtype,
         "layout": "chain" if args.lows is not None else "fixed", "plan": plan_info,
         "compile_us": int(t_parse * 1e6), "sim_us": int(t_sim * 1e6), "norm2": norm2,
         "measured": meas, "p1": {str(q): p for q, p in probs.items()},
-        "samples": [list(map(int, x)) for x in samples],
+        "samples": [list(map(int, x)) for x in samples], "accuracy": accuracy,
     }
 
 
@@ -169,6 +185,8 @@ def main(argv=None) -> int:
     ap.add_argument("--plan", default="file", choices=["file", "auto"],
                     help="auto: the learned cost model picks mode, width and partition (fallback: nat, k = 10)")
     ap.add_argument("--lowering", default="reference", choices=["reference", "textbook"])
+    ap.add_argument("--verify", default="auto", choices=["auto", "none", "fp64", "inverse"],
+                    help="accuracy report: fp64 reference run, inverse circuit, or fp64 for split modes")
     ap.add_argument("--dtype", default="auto", choices=["auto", "complex128", "complex64"],
                     help="state precision (auto: complex128)")
     ap.add_argument("--dot", action="store_true", help="write <file>_PART_<i>.dot")

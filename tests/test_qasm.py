@@ -209,7 +209,7 @@ def _args(**kw):
     import argparse
     base = dict(app="", parts=False, partfile="", nat=0, greedy=0, slots=False, mode="part", layout="fixed",
                 beam="4,2", plan="file", lowering="reference", dtype="auto", dot=False, dump=False, shots=10,
-                seed=0, n=0, alloc=0, initn=0)
+                seed=0, n=0, alloc=0, initn=0, verify="auto")
     base.update(kw)
     return argparse.Namespace(**base)
 
@@ -253,6 +253,32 @@ def test_driver_closed_form_circuits(name, partfile, extra):
     if out["measured"]:
         for shot in out["samples"]:
             assert all(bool((bv_mask(prog) >> q) & 1) == bool(b) for q, b in zip(out["measured"], shot))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,extra,check", [
+    ("bv_n26", {"mode": "fusedbf16", "layout": "chain"}, "auto"),
+    ("qft_n26", {"mode": "fusedbf16", "greedy": 7}, "auto"),
+    ("qft_n26", {"mode": "fused32", "greedy": 5}, "fp64"),
+    ("qf21_n15", {"nat": 8}, "inverse"),
+    ("dnn_n16", {"mode": "fusedbf16", "greedy": 6}, "inverse"),
+    ("qft_n26", {"mode": "fused64", "greedy": 6}, "inverse"),
+])
+def test_driver_accuracy_report(name, extra, check):
+    """The driver's per-run accuracy report: against the FP64 per-gate path
+    in a second state (the default for the split-precision modes), or by the
+    inverse circuit back to |0...0>; the split paths stay within their 1e-6
+    infidelity budget, the FP64 ones within 1e-12."""
+    from hisvsim_amd import driver
+    out = driver.run(_args(app=os.path.join(QDIR, name + ".qasm"), parts=True, verify=check, **extra))
+    acc = out["accuracy"]
+    assert acc is not None and acc["check"] == ("fp64" if check == "auto" else check)
+    tol = 1e-6 if out["mode"] in ("fused32", "fusedbf16") else 1e-12
+    assert acc["infidelity"] <= tol
+    if check == "inverse":
+        assert acc["verified"] and acc["l2_err"] <= (1e-3 if tol == 1e-6 else 1e-6)
+    else:
+        assert acc["max_abs_err"] <= (1e-3 if tol == 1e-6 else 1e-12)
 
 
 @pytest.mark.gpu
