@@ -6,7 +6,7 @@ beside it.
     python bench.py [--steps K] [--warmup W] [--n 33] [--depth 40]
                     [--mode fusedbf16|part|fused64|fused32|pergate|permute] [--k K]
                     [--layout chain|fixed] [--beam W,B] [--dtype complex128|complex64]
-                    [--tune KEY=VALUE ...]
+                    [--plan fixed|auto] [--tune KEY=VALUE ...]
 
 One step = one full simulation of the circuit from |0...0> (state reset
 included) with the chosen path; the state (2^n amplitudes) stays in HBM. The
@@ -18,7 +18,9 @@ per-gate path is timed on a bounded sample of the same circuit (its first
 layers) and its full time-to-solution is extrapolated per gate. Accuracy of
 each path is checked against the FP64 CPU oracle on the same generator at a
 size the oracle finishes in seconds, and a fused path also against the FP64
-part path at the full size. One JSON line is printed last.
+part path at the full size. ``--plan auto`` lets the learned cost model
+choose the path, width and partition among its candidate plans. One JSON
+line is printed last.
 """
 from __future__ import annotations
 
@@ -289,6 +291,9 @@ def main() -> int:
     ap.add_argument("--layout", default="chain", choices=["chain", "fixed"],
                     help="wide BF16-split parts (k >= 8): moving low-slot layout (chain) or qubits 0-4 "
                          "in every part (fixed)")
+    ap.add_argument("--plan", default="fixed", choices=["fixed", "auto"],
+                    help="auto: the learned cost model picks mode, width and partition among its candidate "
+                         "plans (greedy chain planner), instead of --mode/--k/--partitioner/--beam")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="hsv_tune setting for the timed state (repeatable), e.g. part_tiles_per_wg=2")
     ap.add_argument("--no-cpu", action="store_true")
@@ -304,12 +309,26 @@ def main() -> int:
     n, depth = args.n or 33, args.depth
     gates = random_layered(n, depth, seed=0)
     partitioner_used, lows, plan_s = None, None, None
-    if args.mode == "permute":
+    beam = tuple(int(x) for x in args.beam.split(","))
+    if args.plan == "auto" and args.mode not in ("permute", "pergate"):
+        from hisvsim_amd import costmodel
+        tp = time.perf_counter()
+        model = costmodel.MLP.load()
+        try:
+            best, cands = costmodel.choose_plan(model, gates, n)
+        finally:
+            model.close()
+        plan_s = time.perf_counter() - tp
+        args.mode = {"part": "part", "fused64": "fused64", "fusedbf16": "fusedbf16", "chain": "fusedbf16"}[best["mode"]]
+        k, parts, lows = best["k"], best["parts"], best.get("lows")
+        partitioner_used = f"cost model pick among {len(cands)} plans: {costmodel.plan_name(best)}"
+        beam = (1, 1)
+        log(f"[bench] {partitioner_used} (predicted {best['predicted_ms']:.0f} ms), planning {plan_s:.2f} s")
+    elif args.mode == "permute":
         parts = exchange_permutation(n, k)
     elif args.mode == "pergate":
         parts = []
     else:
-        beam = tuple(int(x) for x in args.beam.split(","))
         tp = time.perf_counter()
         parts, partitioner_used, lows = make_parts(gates, args.mode, k, args.partitioner, args.layout, beam)
         plan_s = time.perf_counter() - tp
@@ -477,7 +496,9 @@ def main() -> int:
         "data": "synthetic: seeded random layered circuit (U3 angles ~ U[0,2pi), CZ on random matchings), |0..0> input",
         "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, {args.dtype} state "
                                 f"({2 ** n * S / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),
-                    "mode": args.mode, "k": k, "passes": passes, "partitioner": partitioner_used if lows is None else f"chain (beam {beam[0]}x{beam[1]})",
+                    "mode": args.mode, "k": k, "passes": passes, "partitioner": (partitioner_used if lows is None or args.plan == "auto"
+                                                                      else f"chain (beam {beam[0]}x{beam[1]})"),
+                    "plan": args.plan,
                     # host planning runs once per circuit before the timed
                     # steps, as the reference reads a precomputed parts file:
                     # `value` excludes it

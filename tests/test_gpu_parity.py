@@ -510,7 +510,9 @@ def test_qft_closed_form():
     want = np.empty_like(x)
     want[rev] = np.fft.ifft(x[rev]) * math.sqrt(2 ** n)
     got = gpu_run(n, x, lambda s: s.apply_gates(qft(n)))
-    assert np.abs(got - want).max() <= 1e-11
+    err = np.abs(got - want).max()
+    print(f"QFT18: max error {err:.2e}", flush=True)
+    assert err <= TOL
 
 
 @pytest.mark.parametrize("seed", range(2))
