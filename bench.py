@@ -389,7 +389,7 @@ def main() -> int:
 
         full_verify = None
         if args.mode in FUSED_MODE and not args.no_full_verify and args.dtype == "complex128":
-            # the FP64 register part path on a second full-size state (two
+            # the FP64 part path on a second full-size state (two
             # 2^33 complex128 states fit in HBM together): infidelity of the
             # timed result at the benchmark's own size
             ref_parts, _, _ = make_parts(gates, "part", 12, "best")
@@ -397,7 +397,7 @@ def main() -> int:
                 run_path(ref, "part", gates, ref_parts)
                 ref.synchronize()
                 ov = ref.inner(s)
-                full_verify = {"n": n, "reference": "FP64 register part path (k = 12), same circuit",
+                full_verify = {"n": n, "reference": "FP64 part path (k = 12, tile kernel), same circuit",
                                "infidelity": max(0.0, 1.0 - abs(ov) ** 2 / (ref.norm2() * s.norm2())),
                                "max_abs_err": ref.max_abs_diff(s), "norm2": s.norm2()}
             log(f"[bench] full-size check vs FP64 part path: infidelity {full_verify['infidelity']:.3e}, "

@@ -121,8 +121,9 @@ def test_qft30_bf16split_path():
 
 
 def test_rl33_depth2_part_and_fused_agree():
-    """RL33 (complex128, 137 GB per state) at depth 2: the register part path
-    (parts of 12 including the 5 low qubits) and the fused FP64 path equal
+    """RL33 (complex128, 137 GB per state) at depth 2: the part path (parts
+    of 12 including the 5 low qubits, on the tile kernel) and the fused FP64
+    path equal
     the per-gate path to 1e-12 relative to the largest amplitude on two
     full-size states (amplitudes are ~1e-5 here: an absolute bar of 1e-12
     would pass FP32 arithmetic)."""
