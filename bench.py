@@ -194,7 +194,8 @@ def verify_small(mode: str, k: int, depth: int, n: int = 20, seed: int = 0, part
     with capi.State(n, dtype) as s:
         run_path(s, mode, gates, parts, lows)
         got = s.read()
-    infid = max(0.0, 1.0 - abs(np.vdot(want, got)) ** 2)
+    # normalised overlap: complex64 storage leaves |got|^2 a few 1e-8 off 1
+    infid = max(0.0, 1.0 - abs(np.vdot(want, got)) ** 2 / (np.vdot(want, want).real * np.vdot(got, got).real))
     return {"n": n, "depth": depth, "infidelity": infid, "max_abs_err": float(np.abs(got - want).max())}
 
 
