@@ -4,7 +4,7 @@ on one MI355X, with HBM GB/s of the dominant kernel and the per-gate path
 beside it.
 
     python bench.py [--steps K] [--warmup W] [--n 33] [--depth 40]
-                    [--mode fusedbf16|part|fused64|fused32|pergate|permute] [--k K]
+                    [--mode fusedbf16|fusedbf16x3|part|fused64|fused32|pergate|permute] [--k K]
                     [--layout chain|fixed] [--beam W,B] [--dtype complex128|complex64]
                     [--plan fixed|auto] [--tune KEY=VALUE ...]
 
@@ -61,12 +61,12 @@ def make_parts(gates, mode: str, k: int, partitioner: str = "best", layout: str 
     layout (wide BF16-split parts only) the parts come from chain_parts and
     the low slots follow them (beam: width and branching of the planner's
     search over low sets). Returns (parts, partitioner, lows or None)."""
-    if mode == "fusedbf16" and k >= 8 and layout == "chain":
+    if mode in ("fusedbf16", "fusedbf16x3") and k >= 8 and layout == "chain":
         parts, lows = chain_parts(gates, k, LOW_SLOTS, beam=beam)
         return parts, "chain", lows
     # the register kernel and the wide (k >= 8) BF16-split kernel load whole
     # parts as tiles, so their parts start with the low qubits for long runs
-    always = range(LOW_SLOTS) if mode == "part" or (mode == "fusedbf16" and k >= 8) else ()
+    always = range(LOW_SLOTS) if mode in ("part", "fusedbf16x3") or (mode == "fusedbf16" and k >= 8) else ()
     cands = {"greedy": lambda: greedy_parts(gates, k, always=always),
              "dfs": lambda: dfs_parts(gates, k, always=always),
              "closure": lambda: closure_parts(gates, k, always=always)}
@@ -93,7 +93,7 @@ def measured_traffic(kernel: str, n: int, mode: str, k: int):
     return None if row is None else row.get("bytes_per_launch")
 
 
-FUSED_MODE = {"fused64": "f64", "fused32": "f32split", "fusedbf16": "bf16split"}
+FUSED_MODE = {"fused64": "f64", "fused32": "f32split", "fusedbf16": "bf16split", "fusedbf16x3": "bf16x3"}
 BF16_MFMA_PEAK = 2.5e15
 
 
@@ -279,7 +279,7 @@ def main() -> int:
                                                       "runs in place from 32 qubits on, out of place below)")
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--mode", default="fusedbf16",
-                    choices=["part", "fused64", "fused32", "fusedbf16", "pergate", "permute"])
+                    choices=["part", "fused64", "fused32", "fusedbf16", "fusedbf16x3", "pergate", "permute"])
     ap.add_argument("--k", type=int, default=0, help="part width (default: per mode; permute: qubits exchanged)")
     ap.add_argument("--partitioner", default="best", choices=["best", "greedy", "dfs", "closure"],
                     help="acyclic partitioner (best: the one with fewer parts)")
@@ -306,7 +306,8 @@ def main() -> int:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): sharding the state over several "
               f"GPUs is a design, not built or run here.", file=sys.stderr)
         return 2
-    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 13, "pergate": 1, "permute": 3}[args.mode]
+    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 13, "fusedbf16x3": 12, "pergate": 1,
+                   "permute": 3}[args.mode]
     n, depth = args.n or 33, args.depth
     gates = random_layered(n, depth, seed=0)
     partitioner_used, lows, plan_s = None, None, None
@@ -430,7 +431,7 @@ def main() -> int:
                         "tts_extrapolated_s": pergate_tts, "passes": len(gates),
                         "speedup_of_mode": pergate_tts / tts}
 
-    if args.mode in ("part", "pergate", "permute", "fusedbf16"):
+    if args.mode in ("part", "pergate", "permute", "fusedbf16", "fusedbf16x3"):
         # a BF16-split fused pass executes 3 x 8 x 2^w flops per amplitude for
         # each fused block of width w (one block of k <= 7 qubits, or steps of
         # 5 qubits inside a wide part) against 32 B of HBM traffic: at most
@@ -438,7 +439,7 @@ def main() -> int:
         # over 8 TB/s, so the pass is HBM-bound too
         unit_bytes = 2 * S * 2 ** n
         kernel = {"part": "k_part_reg", "pergate": "k_gate", "permute": "k_permute",
-                  "fusedbf16": "k_part_bf" if k >= 8 else "k_fused_bf"}[args.mode]
+                  "fusedbf16": "k_part_bf" if k >= 8 else "k_fused_bf", "fusedbf16x3": "k_part_bf"}[args.mode]
         launches_per_call = 1
         if args.mode == "part":
             # the tile kernel takes every part that fits one tile with slots
@@ -459,17 +460,20 @@ def main() -> int:
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, n, args.mode, k),
                 "kernel": kernel,
                 "per_launch": f"{unit_bytes} B (2 x {S} B x 2^{n}: read + write of the state) / {kern_ms:.3f} ms"}
-        if args.mode == "fusedbf16":
+        if args.mode in ("fusedbf16", "fusedbf16x3"):
             # blocks per pass: 1 for k <= 7; for wide parts the 5-qubit steps the
-            # host cuts (hsv_part_steps)
+            # host cuts (hsv_part_steps); 3 products per real product with the
+            # two-piece split, 6 with three pieces
             timed = ([p for p, f in zip(parts, chain_full_passes(parts, lows, n)) if f]
                      if lows is not None else parts)   # the passes kern_ms averages
-            blocks = ([len(capi.part_steps(p)) for p in timed] if k >= 8 else [1] * len(timed))
-            width = 5 if k >= 8 else max(5, min(7, k))
+            wide = k >= 8 or args.mode == "fusedbf16x3"
+            blocks = ([len(capi.part_steps(p)) for p in timed] if wide else [1] * len(timed))
+            width = 5 if wide else max(5, min(7, k))
             per_pass = float(np.mean(blocks)) * 8 * 2 ** width * 2 ** n
             roof["fused_blocks_per_pass"] = float(np.mean(blocks))
             roof["mfma_tflops_algorithmic"] = per_pass / (kern_ms * 1e-3) / 1e12
-            roof["mfma_frac_executed"] = 3 * per_pass / (kern_ms * 1e-3) / BF16_MFMA_PEAK
+            products = 6 if args.mode == "fusedbf16x3" else 3
+            roof["mfma_frac_executed"] = products * per_pass / (kern_ms * 1e-3) / BF16_MFMA_PEAK
     else:
         kk = max(4, min(7, k))
         flops = 8 * 2 ** kk * 2 ** n
@@ -493,7 +497,9 @@ def main() -> int:
     out = {
         "metric": METRIC, "value": tts, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": tts * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": {"fused32": "f32split", "fusedbf16": "bf16split (fp32 accumulate)"}.get(args.mode, "f64"),
+        "dtype": {"fused32": "f32split", "fusedbf16": "bf16split (fp32 accumulate)",
+                  "fusedbf16x3": "bf16x3split (three-piece split, fp32-class products, fp32 accumulate)"}.get(
+                      args.mode, "f64"),
         "data": "synthetic: seeded random layered circuit (U3 angles ~ U[0,2pi), CZ on random matchings), |0..0> input",
         "config": ({"workload": f"RL{n} random layered circuit, {n} qubits, depth {depth}, {args.dtype} state "
                                 f"({2 ** n * S / 1e9:.1f} GB)", "n": n, "depth": depth, "gates": len(gates),

@@ -18,9 +18,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhsv.so")
 
 HSV_C128, HSV_C64 = 0, 1
-HSV_FUSED_F64, HSV_FUSED_F32SPLIT, HSV_FUSED_BF16SPLIT = 0, 1, 2
+HSV_FUSED_F64, HSV_FUSED_F32SPLIT, HSV_FUSED_BF16SPLIT, HSV_FUSED_BF16X3 = 0, 1, 2, 3
 CHAIN_RENORMALIZE = 0x100   # HSV_CHAIN_RENORMALIZE
-FUSED_MODES = {"f64": HSV_FUSED_F64, "f32split": HSV_FUSED_F32SPLIT, "bf16split": HSV_FUSED_BF16SPLIT}
+FUSED_MODES = {"f64": HSV_FUSED_F64, "f32split": HSV_FUSED_F32SPLIT, "bf16split": HSV_FUSED_BF16SPLIT,
+               "bf16x3": HSV_FUSED_BF16X3}
 
 # SvSim::GateType numbering (basic_gates.hpp:12-45), as in hsv.h.
 GATE_TYPES = {

@@ -175,6 +175,22 @@ __device__ __forceinline__ void split_pair(float a, float b, uint32_t* hi, uint3
   *lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(rest, bf2v));
 }
 
+// The three-piece split (Henry, Tang and Heinecke, "Leveraging the bfloat16
+// artificial intelligence datatype for higher-precision computations",
+// ARITH 2019): a = hi + mid + lo with each piece the BF16 of what the
+// previous ones leave (both remainders are exact in FP32), 24 significant
+// bits in all; the residual is below 2^-26 |a|.
+__device__ __forceinline__ void split_pair3(float a, float b, uint32_t* hi, uint32_t* mid, uint32_t* lo) {
+  const f2v ab = {a, b};
+  const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(ab, bf2v));
+  const f2v r1 = {a - __builtin_bit_cast(float, h << 16), b - __builtin_bit_cast(float, h & 0xffff0000u)};
+  const uint32_t m = __builtin_bit_cast(uint32_t, __builtin_convertvector(r1, bf2v));
+  const f2v r2 = {r1[0] - __builtin_bit_cast(float, m << 16), r1[1] - __builtin_bit_cast(float, m & 0xffff0000u)};
+  *hi = h;
+  *mid = m;
+  *lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r2, bf2v));
+}
+
 // 8 complex values (rows of a B operand) to the four BF16 operands: real hi,
 // real lo, imaginary hi, imaginary lo.
 __device__ __forceinline__ void split_bf8(const float2 (&v)[8], bf8* rh, bf8* rl, bf8* ih, bf8* il) {
@@ -193,6 +209,30 @@ __device__ __forceinline__ void split_bf8(const float2 (&v)[8], bf8* rh, bf8* rl
   *rl = __builtin_bit_cast(bf8, b);
   *ih = __builtin_bit_cast(bf8, c);
   *il = __builtin_bit_cast(bf8, d);
+}
+
+// The same with three pieces: real hi, mid, lo, imaginary hi, mid, lo.
+__device__ __forceinline__ void split_bf8_3(const float2 (&v)[8], bf8* rh, bf8* rm, bf8* rl, bf8* ih, bf8* im,
+                                            bf8* il) {
+  u4v a, b, c, d, e, f;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint32_t x0, x1, x2, y0, y1, y2;
+    split_pair3(v[2 * j].x, v[2 * j + 1].x, &x0, &x1, &x2);
+    split_pair3(v[2 * j].y, v[2 * j + 1].y, &y0, &y1, &y2);
+    a[j] = x0;
+    b[j] = x1;
+    c[j] = x2;
+    d[j] = y0;
+    e[j] = y1;
+    f[j] = y2;
+  }
+  *rh = __builtin_bit_cast(bf8, a);
+  *rm = __builtin_bit_cast(bf8, b);
+  *rl = __builtin_bit_cast(bf8, c);
+  *ih = __builtin_bit_cast(bf8, d);
+  *im = __builtin_bit_cast(bf8, e);
+  *il = __builtin_bit_cast(bf8, f);
 }
 
 struct BfArgs {
@@ -467,7 +507,10 @@ struct PartBfTables {
 constexpr int part_bf_threads(int kt, bool stream_u) { return kt >= 14 || (kt == 13 && !stream_u) ? 1024 : 512; }
 constexpr int kPartBfMaxSteps = 12;
 
-template <typename R, int KT, bool STREAM_U, bool U_DIRECT = false>
+// NP: BF16 pieces per value, 2 (hi*hi + hi*lo + lo*hi, 16 significant bits)
+// or 3 (the three-piece split: hi*hi, hi*mid, mid*hi, mid*mid, hi*lo, lo*hi,
+// 24 bits, FP32-class products; 12-qubit tiles with resident fragments only).
+template <typename R, int KT, bool STREAM_U, bool U_DIRECT = false, int NP = 2>
 // 16 waves per CU (two workgroups of 8, or one of 16): at most 128 VGPRs
 __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgpu_waves_per_eu(4))) void k_part_bf(typename VecT<R>::type* __restrict__ a,
                                                           const PartBfTables* __restrict__ tabs,
@@ -482,7 +525,8 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
   constexpr int CBW = NCB / W;                // column blocks per wave
   constexpr int PER = T / NTH;
   static_assert(KT == 13 || STREAM_U == (KT >= 14), "12-qubit tiles keep U resident, 14-qubit tiles stream it");
-  constexpr int STEP_FRAGS = 2 * 4 * 64;      // bf8 per step: [rb][set][lane]
+  constexpr int STEP_FRAGS = 2 * 2 * NP * 64;  // bf8 per step: [rb][set][lane]
+  static_assert(NP == 2 || (NP == 3 && KT == 12 && !STREAM_U), "three pieces: 12-qubit tiles, resident fragments");
   static_assert(CBW >= 1 && NCB % W == 0 && PER * NTH == T, "tile shape");
   static_assert(!STREAM_U || STEP_FRAGS <= NTH, "at most one fragment per thread per step");
   static_assert(!U_DIRECT || STREAM_U, "direct fragment loads replace the streaming buffer");
@@ -607,12 +651,79 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
       for (int e = 0; e < 4; ++e) roff2[e] = uint32_t(S.rt[g + 8 * e]) | (uint32_t(S.rt[g + 8 * e + 4]) << 16);
       auto roff = [&](int e) -> uint32_t { return e & 1 ? roff2[e >> 1] >> 16 : roff2[e >> 1] & 0xffffu; };
       uint32_t tc[CBW];
-      bf8 xrh[CBW], xrl[CBW], xih[CBW], xil[CBW];
 #pragma unroll
       for (int w = 0; w < CBW; ++w) {
         const int col = 16 * (wave * CBW + w) + cc;
         tc[w] = S.ctl[col & 15] ^ S.cth[col >> 4];
       }
+      if constexpr (NP == 3) {
+        // three pieces: the six products of each real product, smallest
+        // first into the FP32 accumulator
+        bf8 xrh[CBW], xrm[CBW], xrl[CBW], xih[CBW], xim[CBW], xil[CBW];
+#pragma unroll
+        for (int w = 0; w < CBW; ++w) {
+          float2 v[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) v[e] = ys[roff(e) ^ tc[w]];
+          split_bf8_3(v, &xrh[w], &xrm[w], &xrl[w], &xih[w], &xim[w], &xil[w]);
+        }
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          const bf8* Ar = As + rb * 6 * 64;
+          const bf8 urh = Ar[0], urm = Ar[64], url = Ar[128], uih = Ar[192], uim = Ar[256], uil = Ar[320];
+          v4f_t accR[CBW], accT[CBW], accI[CBW], accJ[CBW];
+#pragma unroll
+          for (int w = 0; w < CBW; ++w) {
+            const v4f_t z = {0, 0, 0, 0};
+            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrl[w], z, 0, 0, 0);
+            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xil[w], z, 0, 0, 0);
+            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xil[w], z, 0, 0, 0);
+            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrl[w], z, 0, 0, 0);
+          }
+#pragma unroll
+          for (int w = 0; w < CBW; ++w) {
+            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xrh[w], accR[w], 0, 0, 0);
+            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xih[w], accT[w], 0, 0, 0);
+            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xih[w], accI[w], 0, 0, 0);
+            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xrh[w], accJ[w], 0, 0, 0);
+          }
+#pragma unroll
+          for (int w = 0; w < CBW; ++w) {
+            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xrm[w], accR[w], 0, 0, 0);
+            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xim[w], accT[w], 0, 0, 0);
+            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xim[w], accI[w], 0, 0, 0);
+            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xrm[w], accJ[w], 0, 0, 0);
+          }
+#pragma unroll
+          for (int w = 0; w < CBW; ++w) {
+            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrm[w], accR[w], 0, 0, 0);
+            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xim[w], accT[w], 0, 0, 0);
+            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xim[w], accI[w], 0, 0, 0);
+            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrm[w], accJ[w], 0, 0, 0);
+          }
+#pragma unroll
+          for (int w = 0; w < CBW; ++w) {
+            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xrh[w], accR[w], 0, 0, 0);
+            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xih[w], accT[w], 0, 0, 0);
+            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xih[w], accI[w], 0, 0, 0);
+            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xrh[w], accJ[w], 0, 0, 0);
+          }
+#pragma unroll
+          for (int w = 0; w < CBW; ++w) {
+            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrh[w], accR[w], 0, 0, 0);
+            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xih[w], accT[w], 0, 0, 0);
+            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xih[w], accI[w], 0, 0, 0);
+            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrh[w], accJ[w], 0, 0, 0);
+          }
+#pragma unroll
+          for (int w = 0; w < CBW; ++w)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              ys[roff(4 * rb + i) ^ tc[w]] = make_float2(accR[w][i] - accT[w][i], accI[w][i] + accJ[w][i]);
+        }
+        continue;
+      }
+      bf8 xrh[CBW], xrl[CBW], xih[CBW], xil[CBW];
 #pragma unroll
       for (int w = 0; w < CBW; ++w) {
         float2 v[8];
@@ -890,12 +1001,12 @@ void choose_part_swizzle(int KT, const std::vector<PartSwizzleKey>& keys, uint32
   }
 }
 
-template <typename R, int KT, bool STREAM_U, bool U_DIRECT = false>
+template <typename R, int KT, bool STREAM_U, bool U_DIRECT = false, int NP = 2>
 void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t steps_off, size_t frag_off,
                     int nsteps, int grid, hipStream_t st, uint64_t ntiles, uint64_t outer, uint64_t top, int tpw,
                     int* rc) {
-  const size_t lds = STREAM_U ? 0 : size_t(nsteps) * 2 * 4 * 64 * sizeof(bf8);
-  auto kern = &k_part_bf<R, KT, STREAM_U, U_DIRECT>;
+  const size_t lds = STREAM_U ? 0 : size_t(nsteps) * 2 * 2 * NP * 64 * sizeof(bf8);
+  auto kern = &k_part_bf<R, KT, STREAM_U, U_DIRECT, NP>;
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)std::max<size_t>(lds, 1)) != hipSuccess) {
     *rc = HSV_ERR_DEVICE;
@@ -912,9 +1023,11 @@ void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t
 // the gates. Logical qubit q sits in slot phys[q] (q when phys is null) and
 // the store leaves it in slot phys_after[q] (unchanged when null): a layout
 // change that must keep every qubit outside the tile where it is.
+// pieces: BF16 pieces per value (2, or 3 for FP32-class products; 3 needs a
+// 12-qubit tile).
 int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
                   const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
-                  const int* phys_after, uint64_t live_slots, const BfTune& bf, std::string* err) {
+                  const int* phys_after, uint64_t live_slots, const BfTune& bf, std::string* err, int pieces) {
   int tpw = bf.part_tpw;
   int ph[64], pa[64], inv[64];
   for (int q = 0; q < n; ++q) ph[q] = phys ? phys[q] : q;
@@ -938,6 +1051,10 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   }
   if (n < KT) {
     *err = "a wide bf16 fused part needs a state of at least " + std::to_string(KT) + " qubits";
+    return HSV_ERR_LIMIT;
+  }
+  if (pieces == 3 && KT != 12) {
+    *err = "the three-piece bf16 split takes parts whose qubits and slots 0-4 fit 12 qubits";
     return HSV_ERR_LIMIT;
   }
   uint64_t tmask = pmask;
@@ -988,7 +1105,9 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
   const int ns = (int)groups.size();
   std::vector<PartBfStep> steps(ns);
   std::vector<PartSwizzleKey> keys(ns);
-  std::vector<uint16_t> frags(size_t(ns) * 2 * 4 * 64 * 8);
+  const int nsets = 2 * pieces;   // real pieces, then imaginary pieces
+  const size_t step_frag_bytes = size_t(2) * nsets * 64 * 16;
+  std::vector<uint16_t> frags(size_t(ns) * 2 * nsets * 64 * 8);
   for (int s = 0; s < ns; ++s) {
     uint64_t gm = 0;
     std::vector<Lowered> sg;
@@ -1027,14 +1146,17 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
           const int row = 16 * rb + 4 * (mm % 4) + mm / 4;
           const int col = lane / 16 + 4 * e;
           const cd u = U[size_t(row) * 32 + col];
-          const uint16_t rh = bf16_rne((float)u.real()), ih = bf16_rne((float)u.imag());
-          const uint16_t rl = bf16_rne((float)(u.real() - bf16_value(rh)));
-          const uint16_t il = bf16_rne((float)(u.imag() - bf16_value(ih)));
-          const size_t f = size_t((s * 2 + rb) * 4) * 64 + lane;
-          frags[(f + 0 * 64) * 8 + e] = rh;
-          frags[(f + 1 * 64) * 8 + e] = rl;
-          frags[(f + 2 * 64) * 8 + e] = ih;
-          frags[(f + 3 * 64) * 8 + e] = il;
+          const size_t f = size_t((s * 2 + rb) * nsets) * 64 + lane;
+          // each piece the BF16 of what the previous ones leave of the
+          // FP64 entry
+          double re = u.real(), im = u.imag();
+          for (int k = 0; k < pieces; ++k) {
+            const uint16_t pr = bf16_rne((float)re), pi = bf16_rne((float)im);
+            frags[(f + size_t(k) * 64) * 8 + e] = pr;
+            frags[(f + size_t(pieces + k) * 64) * 8 + e] = pi;
+            re -= bf16_value(pr);
+            im -= bf16_value(pi);
+          }
         }
   }
   // tile bit j's qubit moves to tile bit sig[j] (the tile position of its new
@@ -1088,10 +1210,10 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
       (size_t(1) << (KT - 4));
   // 13-qubit tiles: resident fragments (1024 threads) when the part's steps
   // fit LDS beside the tile, else streamed
-  const bool stream_u = KT >= 14 || (KT == 13 && (bf.part13 != 1 || (160 * 1024 - static_lds) / (2 * 4 * 64 * 16) < size_t(ns)));
+  const bool stream_u = KT >= 14 || (KT == 13 && (bf.part13 != 1 || (160 * 1024 - static_lds) / step_frag_bytes < size_t(ns)));
   const bool u_direct = KT == 13 && stream_u && bf.part13 == 2;
   const int max_steps = stream_u ? kPartBfMaxSteps
-                                 : (int)std::min<size_t>(kPartBfMaxSteps, (160 * 1024 - static_lds) / (2 * 4 * 64 * 16));
+                                 : (int)std::min<size_t>(kPartBfMaxSteps, (160 * 1024 - static_lds) / step_frag_bytes);
   const size_t steps_off = 2 * sizeof(PartBfTables);
   const size_t frag_off = (steps_off + sizeof(PartBfStep) * ns + 255) & ~size_t(255);
   const size_t bytes = frag_off + frags.size() * 2;
@@ -1122,8 +1244,11 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     const int cnt = std::min(max_steps, ns - s0);
     const size_t to = s0 + cnt >= ns ? sizeof(PartBfTables) : 0;
     const size_t so = steps_off + sizeof(PartBfStep) * s0;
-    const size_t fo = frag_off + size_t(s0) * 2 * 4 * 64 * 16;
-    if (KT == 12) c128 ? launch_part_bf<double, 12, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+    const size_t fo = frag_off + size_t(s0) * step_frag_bytes;
+    if (KT == 12 && pieces == 3)
+      c128 ? launch_part_bf<double, 12, false, false, 3>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+           : launch_part_bf<float, 12, false, false, 3>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
+    else if (KT == 12) c128 ? launch_part_bf<double, 12, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
                        : launch_part_bf<float, 12, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
     else if (KT == 13 && u_direct)
       c128 ? launch_part_bf<double, 13, true, true>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
@@ -1253,15 +1378,29 @@ int fused_apply_bf(void* amp, int n, int dtype, hipStream_t stream, int num_cu, 
 
 int fused_part_bf_layout(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
                          const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
-                         const int* phys_after, uint64_t live_slots, const BfTune& bf, std::string* err) {
-  return part_apply_bf(amp, n, dtype, stream, scratch, gates, tile_qubits, phys, phys_after, live_slots, bf, err);
+                         const int* phys_after, uint64_t live_slots, const BfTune& bf, std::string* err,
+                         int pieces) {
+  return part_apply_bf(amp, n, dtype, stream, scratch, gates, tile_qubits, phys, phys_after, live_slots, bf, err,
+                       pieces);
 }
 
 int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, DeviceScratch* scratch,
                 const std::vector<Lowered>& gates, int mode, std::string* err, const BfTune& bf) {
-  if (mode != HSV_FUSED_F64 && mode != HSV_FUSED_F32SPLIT && mode != HSV_FUSED_BF16SPLIT) {
+  if (mode != HSV_FUSED_F64 && mode != HSV_FUSED_F32SPLIT && mode != HSV_FUSED_BF16SPLIT &&
+      mode != HSV_FUSED_BF16X3) {
     *err = "unknown fused mode";
     return HSV_ERR_ARG;
+  }
+  if (mode == HSV_FUSED_BF16X3) {
+    // the three-piece split runs every part on the wide-part kernel, in a
+    // 12-qubit tile with slots 0-4
+    uint64_t pmask = 0;
+    for (const Lowered& L : gates) {
+      if (L.kind == K_IDENT) continue;
+      for (int i = 0; i < L.nt; ++i) pmask |= 1ull << L.t[i];
+      for (int i = 0; i < L.nc; ++i) pmask |= 1ull << L.c[i];
+    }
+    return part_apply_bf(amp, n, dtype, stream, scratch, gates, pmask, nullptr, nullptr, ~0ull, bf, err, 3);
   }
   if (mode == HSV_FUSED_BF16SPLIT) {
     uint64_t pmask = 0;
@@ -1271,7 +1410,7 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
       for (int i = 0; i < L.nc; ++i) pmask |= 1ull << L.c[i];
     }
     if (__builtin_popcountll(pmask) > 7)
-      return part_apply_bf(amp, n, dtype, stream, scratch, gates, pmask, nullptr, nullptr, ~0ull, bf, err);
+      return part_apply_bf(amp, n, dtype, stream, scratch, gates, pmask, nullptr, nullptr, ~0ull, bf, err, 2);
   }
   std::vector<int> Q;
   std::vector<cd> Um;

@@ -926,7 +926,9 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
     return fail(HSV_ERR_ARG, "bad arguments");
   const bool renorm = (mode & HSV_CHAIN_RENORMALIZE) != 0;
   mode &= ~HSV_CHAIN_RENORMALIZE;
-  if (mode != HSV_FUSED_BF16SPLIT) return fail(HSV_ERR_ARG, "the part chain runs on the BF16-split path only");
+  if (mode != HSV_FUSED_BF16SPLIT && mode != HSV_FUSED_BF16X3)
+    return fail(HSV_ERR_ARG, "the part chain runs on the BF16-split paths only");
+  const int pieces = mode == HSV_FUSED_BF16X3 ? 3 : 2;
   const int n = s->n;
   if (n < 12) return fail(HSV_ERR_LIMIT, "the part chain needs a state of at least 12 qubits");
   long total = 0;
@@ -1055,7 +1057,7 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
     if (int rc = begin_timed(s)) return rc;
     std::string err;
     if (int rc = fused_part_bf_layout(s->amp, n, s->dtype, s->stream, &s->fused_scratch, part, tile[p],
-                                      layout[p].data(), layout[p + 1].data(), live, s->bf, &err)) {
+                                      layout[p].data(), layout[p + 1].data(), live, s->bf, &err, pieces)) {
       // close this part's timing pair, then bring the parts applied so far
       // back to the standard layout; a failed restore is reported too
       std::string msg = "part " + std::to_string(p) + ": " + err;

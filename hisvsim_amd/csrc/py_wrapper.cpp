@@ -378,8 +378,9 @@ PYBIND11_MODULE(svsimulator_py_wrapper, m) {
       [](StateVector& sv, const std::vector<BoundGate*>& gates, const std::string& precision) {
         int mode = precision == "f64" ? HSV_FUSED_F64
                    : precision == "f32split" ? HSV_FUSED_F32SPLIT
-                   : precision == "bf16split" ? HSV_FUSED_BF16SPLIT : -1;
-        if (mode < 0) throw py::value_error("precision must be 'f64', 'f32split' or 'bf16split'");
+                   : precision == "bf16split" ? HSV_FUSED_BF16SPLIT
+                   : precision == "bf16x3" ? HSV_FUSED_BF16X3 : -1;
+        if (mode < 0) throw py::value_error("precision must be 'f64', 'f32split', 'bf16split' or 'bf16x3'");
         auto v = collect(gates);
         py::gil_scoped_release nogil;
         check(hsv_apply_fused(sv.h, v.data(), (int)v.size(), mode), "execute_fused");
@@ -389,7 +390,9 @@ PYBIND11_MODULE(svsimulator_py_wrapper, m) {
   // Not in the reference: the bench's default path from this module. The
   // planner (hsv_chain_parts) cuts a gate list into parts of at most k
   // qubits with a moving set of 5 low qubits; execute_fused_chain runs them
-  // on the BF16-split path (hsv_apply_fused_chain) and renormalises.
+  // on a BF16-split path (hsv_apply_fused_chain; precision "bf16split" or
+  // the three-piece "bf16x3" with parts of at most 12 qubits) and
+  // renormalises.
   m.def(
       "plan_chain",
       [](const std::vector<BoundGate*>& gates, int k, int beam_width, int beam_branch, int lookahead) {
@@ -422,8 +425,11 @@ PYBIND11_MODULE(svsimulator_py_wrapper, m) {
   m.def(
       "execute_fused_chain",
       [](StateVector& sv, const std::vector<std::vector<BoundGate*>>& parts, const std::vector<std::vector<int>>& lows,
-         bool renormalize) {
+         bool renormalize, const std::string& precision) {
         if (lows.size() != parts.size()) throw py::value_error("one low set per part");
+        const int pmode = precision == "bf16split" ? HSV_FUSED_BF16SPLIT
+                          : precision == "bf16x3" ? HSV_FUSED_BF16X3 : -1;
+        if (pmode < 0) throw py::value_error("precision must be 'bf16split' or 'bf16x3'");
         std::vector<hsv_gate> v;
         std::vector<int> sizes, lw;
         for (size_t p = 0; p < parts.size(); ++p) {
@@ -432,12 +438,13 @@ PYBIND11_MODULE(svsimulator_py_wrapper, m) {
           sizes.push_back((int)parts[p].size());
           lw.insert(lw.end(), lows[p].begin(), lows[p].end());
         }
-        const int mode = HSV_FUSED_BF16SPLIT | (renormalize ? HSV_CHAIN_RENORMALIZE : 0);
+        const int mode = pmode | (renormalize ? HSV_CHAIN_RENORMALIZE : 0);
         py::gil_scoped_release nogil;
         check(hsv_apply_fused_chain(sv.h, v.data(), (int)v.size(), sizes.data(), (int)sizes.size(), lw.data(), mode),
               "execute_fused_chain");
       },
-      py::arg("sv"), py::arg("parts"), py::arg("lows"), py::arg("renormalize") = true);
+      py::arg("sv"), py::arg("parts"), py::arg("lows"), py::arg("renormalize") = true,
+      py::arg("precision") = "bf16split");
 
   m.def("obtain_apply_time", []() { long long a; hsv_get_counters(&a, nullptr, nullptr, nullptr); return a; });
   m.def("obtain_gate_counter", []() { long long g; hsv_get_counters(nullptr, &g, nullptr, nullptr); return g; });

@@ -6,7 +6,7 @@ Counterpart of the reference driver's main flow and part executor
 ``execute_qasm`` :1377-1617 for the per-gate path):
 
     python -m hisvsim_amd.driver --app circuit.qasm [--parts --partfile F]
-        [--nat K | --greedy K] [--slots] [--mode part|fused64|fused32|fusedbf16|pergate]
+        [--nat K | --greedy K] [--slots] [--mode part|fused64|fused32|fusedbf16|fusedbf16x3|pergate]
         [--layout fixed|chain [--beam W,B]] [--plan file|auto]
         [--lowering reference|textbook] [--verify auto|none|fp64|inverse]
         [--dot] [--dump] [--shots N]
@@ -20,6 +20,9 @@ Without ``--parts`` every gate is applied on its own (the per-gate path).
 ``--mode fusedbf16 --layout chain`` runs the benchmark's path: parts of up
 to ``--nat``/``--greedy`` K (default 13) qubits from the moving-layout
 planner (``chain_parts``, beam ``--beam``) through hsv_apply_fused_chain.
+``--mode fusedbf16x3`` is the three-piece BF16 split (FP32-class products;
+parts whose qubits and qubits 0-4 fit 12, with ``--layout chain`` K
+defaults to 12).
 ``--plan auto`` lets the learned cost model (``costmodel.choose_plan``)
 pick the mode, width and partition among its candidates; without a trained
 parameter file (or device for the network) it falls back to the plain
@@ -73,7 +76,8 @@ def plan_parts(prog: qasm.QasmProgram, args) -> List[list]:
     if args.layout == "chain":
         from .partition import chain_parts
         beam = tuple(int(x) for x in args.beam.split(","))
-        parts, lows = chain_parts(gates, args.nat or args.greedy or 13, 5, beam=beam)
+        parts, lows = chain_parts(gates, args.nat or args.greedy or (12 if args.mode == "fusedbf16x3" else 13), 5,
+                                  beam=beam)
         args.lows = lows
         return parts
     if args.partfile:
@@ -138,9 +142,10 @@ def run(args) -> dict:
             for p in parts:
                 s.apply_part(p, 1 if args.slots else 0)
         elif args.lows is not None:
-            s.apply_fused_chain(parts, args.lows, "bf16split", renormalize=True)
+            s.apply_fused_chain(parts, args.lows, "bf16x3" if args.mode == "fusedbf16x3" else "bf16split",
+                                renormalize=True)
         else:
-            fm = {"fused64": "f64", "fused32": "f32split", "fusedbf16": "bf16split"}[args.mode]
+            fm = {"fused64": "f64", "fused32": "f32split", "fusedbf16": "bf16split", "fusedbf16x3": "bf16x3"}[args.mode]
             for p in parts:
                 s.apply_fused(p, fm)
             if fm != "f64":
@@ -154,7 +159,7 @@ def run(args) -> dict:
         mode = args.mode if args.parts else "pergate"
         check = getattr(args, "verify", "auto")
         if check == "auto":
-            check = "fp64" if mode in ("fused32", "fusedbf16") else "none"
+            check = "fp64" if mode in ("fused32", "fusedbf16", "fusedbf16x3") else "none"
         accuracy = None
         if check == "fp64":
             accuracy = verify.against_fp64(s, gates)
@@ -178,9 +183,11 @@ def main(argv=None) -> int:
     ap.add_argument("--nat", type=int, default=0, help="natural-order parts of K qubits")
     ap.add_argument("--greedy", type=int, default=0, help="DAG-aware greedy parts of K qubits")
     ap.add_argument("--slots", action="store_true", help="add the fast low qubits to every part")
-    ap.add_argument("--mode", default="part", choices=["part", "fused64", "fused32", "fusedbf16", "pergate"])
+    ap.add_argument("--mode", default="part",
+                    choices=["part", "fused64", "fused32", "fusedbf16", "fusedbf16x3", "pergate"])
     ap.add_argument("--layout", default="fixed", choices=["fixed", "chain"],
-                    help="chain: moving low-slot layout (fusedbf16 only; parts of --nat/--greedy K, default 13)")
+                    help="chain: moving low-slot layout (fusedbf16 / fusedbf16x3; parts of --nat/--greedy K, "
+                         "default 13 / 12)")
     ap.add_argument("--beam", default="16,8", metavar="WIDTH,BRANCH", help="chain planner beam (1,1 = greedy)")
     ap.add_argument("--plan", default="file", choices=["file", "auto"],
                     help="auto: the learned cost model picks mode, width and partition (fallback: nat, k = 10)")
@@ -197,8 +204,8 @@ def main(argv=None) -> int:
     ap.add_argument("--alloc", type=int, default=0, help="allocation scheme (no effect on the GPU path)")
     ap.add_argument("--initn", type=int, default=0, help="init threads (no effect on the GPU path)")
     args = ap.parse_args(argv)
-    if args.layout == "chain" and (args.mode != "fusedbf16" or not args.parts):
-        ap.error("--layout chain runs partitioned on the BF16-split path: add --parts --mode fusedbf16")
+    if args.layout == "chain" and (args.mode not in ("fusedbf16", "fusedbf16x3") or not args.parts):
+        ap.error("--layout chain runs partitioned on a BF16-split path: add --parts --mode fusedbf16")
     print(json.dumps(run(args)), flush=True)
     return 0
 

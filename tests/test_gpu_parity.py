@@ -169,6 +169,77 @@ def test_fused_bf16split_wide_parts(orc, k, n, ngates, seed, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("k,n,ngates,seed", [(3, 12, 12, 1), (5, 14, 30, 2), (7, 16, 60, 3), (8, 14, 40, 4),
+                                             (12, 16, 90, 5), (12, 20, 400, 6), (10, 18, 80, 7)])
+def test_fused_bf16x3_parts(orc, k, n, ngates, seed, dtype):
+    """The three-piece BF16 split (HSV_FUSED_BF16X3: six products per real
+    product, 24 significant bits per operand, FP32 accumulation): parts of
+    3..12 qubits, each in a 12-qubit tile with slots 0-4 (the 400-gate part
+    runs in several passes). No renormalisation needed: infidelity within
+    1e-11 and max error within 1e-6 relative of the FP64 oracle (FP32 state
+    in LDS per pass; the two-piece split measures ~1e-9 on these), the norm
+    within 1e-6."""
+    rng = np.random.default_rng(1900 + seed)
+    parts = random_parts(rng, n, 3, min(k, 7), ngates // 3, names=UNITARY_ONE + UNITARY_TWO, max_controls=2)
+    if k > 7:
+        lows = [q for q in range(5)]
+        qs = lows + [int(q) for q in rng.choice(range(5, n), k - 5, replace=False)]
+        parts = []
+        for _ in range(3):
+            part = []
+            for _ in range(ngates // 3):
+                g = random_circuit(rng, k, 1, names=UNITARY_ONE + UNITARY_TWO, max_controls=2)[0]
+                part.append(Gate(g.name, tuple(qs[t] for t in g.targets), tuple(qs[c] for c in g.controls),
+                                 g.params))
+            parts.append(part)
+    a = rand_state(n, seed)
+    want = a.copy()
+    for p in parts:
+        orc.apply_gates(want, p)
+    got = gpu_run(n, a, lambda s: [s.apply_fused(p, "bf16x3") for p in parts], dtype)
+    scale = np.abs(want).max()
+    infid = 1 - abs(np.vdot(want, got)) ** 2 / np.vdot(got, got).real
+    print(f"bf16x3 k={k} n={n}: infidelity {infid:.2e}, max error {np.abs(got - want).max() / scale:.2e} relative",
+          flush=True)
+    assert infid <= 1e-11
+    assert np.abs(got - want).max() <= 1e-6 * scale
+    assert abs(np.vdot(got, got).real - 1) <= 1e-6
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("n,depth", [(16, 12), (20, 14)])
+def test_fused_chain_bf16x3(orc, n, depth, dtype):
+    """The moving-layout chain at k = 12 with the three-piece split: within
+    infidelity 1e-11 of the FP64 oracle, and at least 100 times closer than
+    the two-piece split on the same parts."""
+    from hisvsim_amd.partition import chain_parts
+    gates = random_layered(n, depth, seed=3 * n + depth)
+    parts, lows = chain_parts(gates, 12)
+    a = rand_state(n, depth)
+    want = a.copy()
+    orc.apply_gates(want, gates)
+    infid = {}
+    for mode in ("bf16x3", "bf16split"):
+        got = gpu_run(n, a, lambda s: [s.apply_fused_chain(parts, lows, mode, renormalize=True)], dtype)
+        infid[mode] = max(1 - abs(np.vdot(want, got)) ** 2 / np.vdot(got, got).real, 1e-18)
+        if mode == "bf16x3":
+            assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+    print(f"chain n={n}: infidelity bf16x3 {infid['bf16x3']:.2e}, bf16split {infid['bf16split']:.2e}", flush=True)
+    assert infid["bf16x3"] <= 1e-11
+    assert infid["bf16x3"] * 100 <= infid["bf16split"]
+
+
+def test_fused_bf16x3_refuses_parts_wider_than_its_tile():
+    """A 10-qubit part off qubits 0-4 needs a 13-qubit tile: the three-piece
+    split (12-qubit tiles only) refuses it."""
+    n = 16
+    gates = [Gate("H", (q,)) for q in range(6, 16)]
+    with capi.State(n) as s:
+        with pytest.raises(capi.HsvError):
+            s.apply_fused(gates, "bf16x3")
+
+
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
 def test_fused_bf16split_wide_part_off_the_low_qubits(orc, dtype):
     """An 11-qubit part on qubits 2..12: the tile adds qubits 0 and 1 so that
     it holds qubits 0-4 (a 13-qubit tile), and the steps pad with them."""
