@@ -306,7 +306,7 @@ def main() -> int:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): sharding the state over several "
               f"GPUs is a design, not built or run here.", file=sys.stderr)
         return 2
-    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 13, "fusedbf16x3": 12, "pergate": 1,
+    k = args.k or {"part": 12, "fused64": 5, "fused32": 5, "fusedbf16": 13, "fusedbf16x3": 13, "pergate": 1,
                    "permute": 3}[args.mode]
     n, depth = args.n or 33, args.depth
     gates = random_layered(n, depth, seed=0)

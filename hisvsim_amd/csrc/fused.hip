@@ -506,10 +506,15 @@ struct PartBfTables {
 // qubits): it saves passes only where parts get much wider.
 constexpr int part_bf_threads(int kt, bool stream_u) { return kt >= 14 || (kt == 13 && !stream_u) ? 1024 : 512; }
 constexpr int kPartBfMaxSteps = 12;
+// steps per launch: 8 for the streamed three-piece 13-qubit form (its 12 KB
+// fragment buffer leaves room for 8 step tables, two workgroups per CU)
+constexpr int part_bf_max_steps(int kt, int np) { return np == 3 && kt == 13 ? 8 : kPartBfMaxSteps; }
 
 // NP: BF16 pieces per value, 2 (hi*hi + hi*lo + lo*hi, 16 significant bits)
 // or 3 (the three-piece split: hi*hi, hi*mid, mid*hi, mid*mid, hi*lo, lo*hi,
-// 24 bits, FP32-class products; 12-qubit tiles with resident fragments only).
+// 24 bits, FP32-class products; 12-qubit tiles with resident fragments, or
+// 13-qubit tiles streaming 12 KB of fragments per step, with at most 8 steps
+// per launch so that two workgroups still fit a CU).
 template <typename R, int KT, bool STREAM_U, bool U_DIRECT = false, int NP = 2>
 // 16 waves per CU (two workgroups of 8, or one of 16): at most 128 VGPRs
 __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgpu_waves_per_eu(4))) void k_part_bf(typename VecT<R>::type* __restrict__ a,
@@ -526,9 +531,12 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
   constexpr int PER = T / NTH;
   static_assert(KT == 13 || STREAM_U == (KT >= 14), "12-qubit tiles keep U resident, 14-qubit tiles stream it");
   constexpr int STEP_FRAGS = 2 * 2 * NP * 64;  // bf8 per step: [rb][set][lane]
-  static_assert(NP == 2 || (NP == 3 && KT == 12 && !STREAM_U), "three pieces: 12-qubit tiles, resident fragments");
+  static_assert(NP == 2 || (NP == 3 && ((KT == 12 && !STREAM_U) || (KT == 13 && STREAM_U && !U_DIRECT))),
+                "three pieces: 12-qubit tiles with resident fragments or streamed 13-qubit tiles");
+  constexpr int FRPT = (STEP_FRAGS + NTH - 1) / NTH;  // streamed fragments held per thread
+  constexpr int MAX_STEPS = part_bf_max_steps(KT, NP);
   static_assert(CBW >= 1 && NCB % W == 0 && PER * NTH == T, "tile shape");
-  static_assert(!STREAM_U || STEP_FRAGS <= NTH, "at most one fragment per thread per step");
+  static_assert(!STREAM_U || STEP_FRAGS <= 2 * NTH, "at most two fragments per thread per step");
   static_assert(!U_DIRECT || STREAM_U, "direct fragment loads replace the streaming buffer");
   constexpr bool SU_BUF = STREAM_U && !U_DIRECT;
   using V = typename VecT<R>::type;
@@ -537,7 +545,7 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
   __shared__ bf8 su_one[SU_BUF ? STEP_FRAGS : 1];
   bf8* su = STREAM_U ? su_one : reinterpret_cast<bf8*>(smem_raw);     // [step][rb][set][lane]
   __shared__ float2 ys[T];
-  __shared__ PartBfStep st[kPartBfMaxSteps];
+  __shared__ PartBfStep st[MAX_STEPS];
   __shared__ uint64_t alo[64], ahi[T >> 6];
   __shared__ uint16_t plo[64], phi[T >> 6];
   __shared__ uint8_t swz[T >> 4];
@@ -564,9 +572,12 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
   const uint64_t t0 = uint64_t(blockIdx.x) * tiles_per_wg;
   if (t0 >= ntiles) return;  // uniform across the workgroup
   const int nt = (int)min(uint64_t(tiles_per_wg), ntiles - t0);
-  bf8 fr;  // streamed fragments: this thread's share of the next step
-  if constexpr (SU_BUF)
-    if (threadIdx.x < STEP_FRAGS) fr = ufrag[threadIdx.x];
+  bf8 fr[FRPT];  // streamed fragments: this thread's share of the next step
+  if constexpr (SU_BUF) {
+#pragma unroll
+    for (int i = 0; i < FRPT; ++i)
+      if (threadIdx.x + NTH * i < STEP_FRAGS) fr[i] = ufrag[threadIdx.x + NTH * i];
+  }
   __syncthreads();  // tables ready
   // Thread x handles tile positions t = x + NTH j: t & 63 is its lane and
   // t >> 6 = wave + W j, so the per-j table entries sit at fixed offsets
@@ -630,9 +641,13 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
       const PartBfStep& S = st[s];
       if constexpr (U_DIRECT) {
       } else if constexpr (STREAM_U) {
-        if (threadIdx.x < STEP_FRAGS) {
-          su_one[threadIdx.x] = fr;  // every wave is past the previous step's reads
-          fr = ufrag[(s + 1 < nsteps ? s + 1 : 0) * STEP_FRAGS + threadIdx.x];
+#pragma unroll
+        for (int i = 0; i < FRPT; ++i) {
+          const int f = threadIdx.x + NTH * i;
+          if (f < STEP_FRAGS) {
+            su_one[f] = fr[i];  // every wave is past the previous step's reads
+            fr[i] = ufrag[(s + 1 < nsteps ? s + 1 : 0) * STEP_FRAGS + f];
+          }
         }
         __syncthreads();
         As = su_one + lane;
@@ -658,68 +673,52 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
       }
       if constexpr (NP == 3) {
         // three pieces: the six products of each real product, smallest
-        // first into the FP32 accumulator
-        bf8 xrh[CBW], xrm[CBW], xrl[CBW], xih[CBW], xim[CBW], xil[CBW];
+        // first into the FP32 accumulator; one column block at a time (each
+        // wave owns its columns, so a block is read, multiplied and written
+        // before the next without a hazard), which keeps the eighteen BF16
+        // operands of a block and its accumulators within 128 VGPRs
 #pragma unroll
         for (int w = 0; w < CBW; ++w) {
-          float2 v[8];
+          bf8 xrh, xrm, xrl, xih, xim, xil;
+          {
+            float2 v[8];
 #pragma unroll
-          for (int e = 0; e < 8; ++e) v[e] = ys[roff(e) ^ tc[w]];
-          split_bf8_3(v, &xrh[w], &xrm[w], &xrl[w], &xih[w], &xim[w], &xil[w]);
-        }
+            for (int e = 0; e < 8; ++e) v[e] = ys[roff(e) ^ tc[w]];
+            split_bf8_3(v, &xrh, &xrm, &xrl, &xih, &xim, &xil);
+          }
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-          const bf8* Ar = As + rb * 6 * 64;
-          const bf8 urh = Ar[0], urm = Ar[64], url = Ar[128], uih = Ar[192], uim = Ar[256], uil = Ar[320];
-          v4f_t accR[CBW], accT[CBW], accI[CBW], accJ[CBW];
-#pragma unroll
-          for (int w = 0; w < CBW; ++w) {
+          for (int rb = 0; rb < 2; ++rb) {
+            const bf8* Ar = As + rb * 6 * 64;
+            const bf8 urh = Ar[0], urm = Ar[64], url = Ar[128], uih = Ar[192], uim = Ar[256], uil = Ar[320];
             const v4f_t z = {0, 0, 0, 0};
-            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrl[w], z, 0, 0, 0);
-            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xil[w], z, 0, 0, 0);
-            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xil[w], z, 0, 0, 0);
-            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrl[w], z, 0, 0, 0);
-          }
-#pragma unroll
-          for (int w = 0; w < CBW; ++w) {
-            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xrh[w], accR[w], 0, 0, 0);
-            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xih[w], accT[w], 0, 0, 0);
-            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xih[w], accI[w], 0, 0, 0);
-            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xrh[w], accJ[w], 0, 0, 0);
-          }
-#pragma unroll
-          for (int w = 0; w < CBW; ++w) {
-            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xrm[w], accR[w], 0, 0, 0);
-            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xim[w], accT[w], 0, 0, 0);
-            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xim[w], accI[w], 0, 0, 0);
-            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xrm[w], accJ[w], 0, 0, 0);
-          }
-#pragma unroll
-          for (int w = 0; w < CBW; ++w) {
-            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrm[w], accR[w], 0, 0, 0);
-            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xim[w], accT[w], 0, 0, 0);
-            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xim[w], accI[w], 0, 0, 0);
-            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrm[w], accJ[w], 0, 0, 0);
-          }
-#pragma unroll
-          for (int w = 0; w < CBW; ++w) {
-            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xrh[w], accR[w], 0, 0, 0);
-            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xih[w], accT[w], 0, 0, 0);
-            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xih[w], accI[w], 0, 0, 0);
-            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xrh[w], accJ[w], 0, 0, 0);
-          }
-#pragma unroll
-          for (int w = 0; w < CBW; ++w) {
-            accR[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrh[w], accR[w], 0, 0, 0);
-            accT[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xih[w], accT[w], 0, 0, 0);
-            accI[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xih[w], accI[w], 0, 0, 0);
-            accJ[w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrh[w], accJ[w], 0, 0, 0);
-          }
-#pragma unroll
-          for (int w = 0; w < CBW; ++w)
+            v4f_t accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrl, z, 0, 0, 0);
+            v4f_t accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xil, z, 0, 0, 0);
+            v4f_t accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xil, z, 0, 0, 0);
+            v4f_t accJ = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrl, z, 0, 0, 0);
+            accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xrh, accR, 0, 0, 0);
+            accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xih, accT, 0, 0, 0);
+            accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(url, xih, accI, 0, 0, 0);
+            accJ = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uil, xrh, accJ, 0, 0, 0);
+            accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xrm, accR, 0, 0, 0);
+            accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xim, accT, 0, 0, 0);
+            accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xim, accI, 0, 0, 0);
+            accJ = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xrm, accJ, 0, 0, 0);
+            accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrm, accR, 0, 0, 0);
+            accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xim, accT, 0, 0, 0);
+            accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xim, accI, 0, 0, 0);
+            accJ = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrm, accJ, 0, 0, 0);
+            accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xrh, accR, 0, 0, 0);
+            accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xih, accT, 0, 0, 0);
+            accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urm, xih, accI, 0, 0, 0);
+            accJ = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uim, xrh, accJ, 0, 0, 0);
+            accR = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xrh, accR, 0, 0, 0);
+            accT = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xih, accT, 0, 0, 0);
+            accI = __builtin_amdgcn_mfma_f32_16x16x32_bf16(urh, xih, accI, 0, 0, 0);
+            accJ = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrh, accJ, 0, 0, 0);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-              ys[roff(4 * rb + i) ^ tc[w]] = make_float2(accR[w][i] - accT[w][i], accI[w][i] + accJ[w][i]);
+              ys[roff(4 * rb + i) ^ tc[w]] = make_float2(accR[i] - accT[i], accI[i] + accJ[i]);
+          }
         }
         continue;
       }
@@ -1023,8 +1022,8 @@ void launch_part_bf(void* amp, const unsigned char* dev, size_t tabs_off, size_t
 // the gates. Logical qubit q sits in slot phys[q] (q when phys is null) and
 // the store leaves it in slot phys_after[q] (unchanged when null): a layout
 // change that must keep every qubit outside the tile where it is.
-// pieces: BF16 pieces per value (2, or 3 for FP32-class products; 3 needs a
-// 12-qubit tile).
+// pieces: BF16 pieces per value (2, or 3 for FP32-class products; 3 takes
+// 12- and 13-qubit tiles).
 int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
                   const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
                   const int* phys_after, uint64_t live_slots, const BfTune& bf, std::string* err, int pieces) {
@@ -1053,8 +1052,8 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     *err = "a wide bf16 fused part needs a state of at least " + std::to_string(KT) + " qubits";
     return HSV_ERR_LIMIT;
   }
-  if (pieces == 3 && KT != 12) {
-    *err = "the three-piece bf16 split takes parts whose qubits and slots 0-4 fit 12 qubits";
+  if (pieces == 3 && KT > 13) {
+    *err = "the three-piece bf16 split takes parts whose qubits and slots 0-4 fit 13 qubits";
     return HSV_ERR_LIMIT;
   }
   uint64_t tmask = pmask;
@@ -1210,9 +1209,11 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
       (size_t(1) << (KT - 4));
   // 13-qubit tiles: resident fragments (1024 threads) when the part's steps
   // fit LDS beside the tile, else streamed
-  const bool stream_u = KT >= 14 || (KT == 13 && (bf.part13 != 1 || (160 * 1024 - static_lds) / step_frag_bytes < size_t(ns)));
-  const bool u_direct = KT == 13 && stream_u && bf.part13 == 2;
-  const int max_steps = stream_u ? kPartBfMaxSteps
+  // (the three-piece split streams them at 13 qubits)
+  const bool stream_u = KT >= 14 || (KT == 13 && (pieces == 3 || bf.part13 != 1 ||
+                                                  (160 * 1024 - static_lds) / step_frag_bytes < size_t(ns)));
+  const bool u_direct = KT == 13 && stream_u && bf.part13 == 2 && pieces == 2;
+  const int max_steps = stream_u ? part_bf_max_steps(KT, pieces)
                                  : (int)std::min<size_t>(kPartBfMaxSteps, (160 * 1024 - static_lds) / step_frag_bytes);
   const size_t steps_off = 2 * sizeof(PartBfTables);
   const size_t frag_off = (steps_off + sizeof(PartBfStep) * ns + 255) & ~size_t(255);
@@ -1248,6 +1249,9 @@ int part_apply_bf(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch
     if (KT == 12 && pieces == 3)
       c128 ? launch_part_bf<double, 12, false, false, 3>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
            : launch_part_bf<float, 12, false, false, 3>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
+    else if (KT == 13 && pieces == 3)
+      c128 ? launch_part_bf<double, 13, true, false, 3>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
+           : launch_part_bf<float, 13, true, false, 3>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
     else if (KT == 12) c128 ? launch_part_bf<double, 12, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc)
                        : launch_part_bf<float, 12, false>(amp, dev, to, so, fo, cnt, grid, stream, ntiles, outer, top, tpw, &rc);
     else if (KT == 13 && u_direct)
@@ -1393,7 +1397,7 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
   }
   if (mode == HSV_FUSED_BF16X3) {
     // the three-piece split runs every part on the wide-part kernel, in a
-    // 12-qubit tile with slots 0-4
+    // 12- or 13-qubit tile with slots 0-4
     uint64_t pmask = 0;
     for (const Lowered& L : gates) {
       if (L.kind == K_IDENT) continue;

@@ -53,7 +53,7 @@ int fused_apply(void* amp, int n, int dtype, hipStream_t stream, int num_cu, Dev
 // nonzero amplitude (the others hold |0> on the whole state, so tiles with
 // such a bit set are zero and are skipped); ~0 when nothing is known.
 // pieces: BF16 pieces per value, 2 (HSV_FUSED_BF16SPLIT) or 3
-// (HSV_FUSED_BF16X3, 12-qubit tiles only).
+// (HSV_FUSED_BF16X3, 12- and 13-qubit tiles).
 int fused_part_bf_layout(void* amp, int n, int dtype, hipStream_t stream, DeviceScratch* scratch,
                          const std::vector<Lowered>& gates, uint64_t tile_qubits, const int* phys,
                          const int* phys_after, uint64_t live_slots, const BfTune& bf, std::string* err,

@@ -21,8 +21,7 @@ Without ``--parts`` every gate is applied on its own (the per-gate path).
 to ``--nat``/``--greedy`` K (default 13) qubits from the moving-layout
 planner (``chain_parts``, beam ``--beam``) through hsv_apply_fused_chain.
 ``--mode fusedbf16x3`` is the three-piece BF16 split (FP32-class products;
-parts whose qubits and qubits 0-4 fit 12, with ``--layout chain`` K
-defaults to 12).
+parts whose qubits and qubits 0-4 fit 13).
 ``--plan auto`` lets the learned cost model (``costmodel.choose_plan``)
 pick the mode, width and partition among its candidates; without a trained
 parameter file (or device for the network) it falls back to the plain
@@ -76,8 +75,7 @@ def plan_parts(prog: qasm.QasmProgram, args) -> List[list]:
     if args.layout == "chain":
         from .partition import chain_parts
         beam = tuple(int(x) for x in args.beam.split(","))
-        parts, lows = chain_parts(gates, args.nat or args.greedy or (12 if args.mode == "fusedbf16x3" else 13), 5,
-                                  beam=beam)
+        parts, lows = chain_parts(gates, args.nat or args.greedy or 13, 5, beam=beam)
         args.lows = lows
         return parts
     if args.partfile:
@@ -187,7 +185,7 @@ def main(argv=None) -> int:
                     choices=["part", "fused64", "fused32", "fusedbf16", "fusedbf16x3", "pergate"])
     ap.add_argument("--layout", default="fixed", choices=["fixed", "chain"],
                     help="chain: moving low-slot layout (fusedbf16 / fusedbf16x3; parts of --nat/--greedy K, "
-                         "default 13 / 12)")
+                         "default 13)")
     ap.add_argument("--beam", default="16,8", metavar="WIDTH,BRANCH", help="chain planner beam (1,1 = greedy)")
     ap.add_argument("--plan", default="file", choices=["file", "auto"],
                     help="auto: the learned cost model picks mode, width and partition (fallback: nat, k = 10)")

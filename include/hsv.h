@@ -71,7 +71,7 @@ enum hsv_fused_mode {
   HSV_FUSED_BF16X3 = 3,     /* BF16 matrix cores, three-piece split (six
                                products, 24 significant bits: FP32-class
                                compensated products) accumulated in FP32;
-                               parts whose qubits and qubits 0-4 fit 12 */
+                               parts whose qubits and qubits 0-4 fit 13 */
   /* hsv_apply_fused_chain only, OR-ed into the mode: end with the state
    * renormalised (as hsv_renormalize, but folded into the layout-restoring
    * swap passes when there are two; a zero state stays as it is). */

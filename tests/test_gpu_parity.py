@@ -170,12 +170,15 @@ def test_fused_bf16split_wide_parts(orc, k, n, ngates, seed, dtype):
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
 @pytest.mark.parametrize("k,n,ngates,seed", [(3, 12, 12, 1), (5, 14, 30, 2), (7, 16, 60, 3), (8, 14, 40, 4),
-                                             (12, 16, 90, 5), (12, 20, 400, 6), (10, 18, 80, 7)])
+                                             (12, 16, 90, 5), (12, 20, 400, 6), (10, 18, 80, 7), (13, 16, 120, 8),
+                                             (13, 13, 60, 9), (13, 20, 500, 10)])
 def test_fused_bf16x3_parts(orc, k, n, ngates, seed, dtype):
     """The three-piece BF16 split (HSV_FUSED_BF16X3: six products per real
     product, 24 significant bits per operand, FP32 accumulation): parts of
-    3..12 qubits, each in a 12-qubit tile with slots 0-4 (the 400-gate part
-    runs in several passes). No renormalisation needed: infidelity within
+    3..13 qubits, each in a 12-qubit tile with slots 0-4 (fragments resident
+    in LDS) or a 13-qubit one (fragments streamed; at most 8 steps a launch,
+    so the 400- and 500-gate parts run in several passes). No
+    renormalisation needed: infidelity within
     1e-11 and max error within 1e-6 relative of the FP64 oracle (FP32 state
     in LDS per pass; the two-piece split measures ~1e-9 on these), the norm
     within 1e-6."""
@@ -207,14 +210,14 @@ def test_fused_bf16x3_parts(orc, k, n, ngates, seed, dtype):
 
 
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
-@pytest.mark.parametrize("n,depth", [(16, 12), (20, 14)])
-def test_fused_chain_bf16x3(orc, n, depth, dtype):
-    """The moving-layout chain at k = 12 with the three-piece split: within
-    infidelity 1e-11 of the FP64 oracle, and at least 100 times closer than
-    the two-piece split on the same parts."""
+@pytest.mark.parametrize("n,depth,k", [(16, 12, 12), (20, 14, 12), (18, 12, 13), (20, 14, 13)])
+def test_fused_chain_bf16x3(orc, n, depth, k, dtype):
+    """The moving-layout chain at k = 12 and 13 with the three-piece split:
+    within infidelity 1e-11 of the FP64 oracle, and at least 100 times
+    closer than the two-piece split on the same parts."""
     from hisvsim_amd.partition import chain_parts
     gates = random_layered(n, depth, seed=3 * n + depth)
-    parts, lows = chain_parts(gates, 12)
+    parts, lows = chain_parts(gates, k)
     a = rand_state(n, depth)
     want = a.copy()
     orc.apply_gates(want, gates)
@@ -230,8 +233,8 @@ def test_fused_chain_bf16x3(orc, n, depth, dtype):
 
 
 def test_fused_bf16x3_refuses_parts_wider_than_its_tile():
-    """A 10-qubit part off qubits 0-4 needs a 13-qubit tile: the three-piece
-    split (12-qubit tiles only) refuses it."""
+    """A 10-qubit part off qubits 0-4 needs a 14-qubit tile: the three-piece
+    split (12- and 13-qubit tiles) refuses it."""
     n = 16
     gates = [Gate("H", (q,)) for q in range(6, 16)]
     with capi.State(n) as s:
