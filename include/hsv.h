@@ -138,8 +138,8 @@ int hsv_apply_fused_chain(hsv_state* s, const hsv_gate* gates, int num_gates, co
  * renormalisation's reduction partials and one part's fused-step tables and
  * BF16 fragments) is a static 256 KiB array per GPU, held until the process
  * ends; a part that needs more than its 240 KiB of step data (more than 28
- * fused 5-qubit steps) is refused with HSV_ERR_LIMIT before any pass of
- * that part runs. Every call on one GPU shares that GPU's array: calls that
+ * fused 5-qubit steps; 19 with HSV_FUSED_BF16X3, whose fragments take 12 KB
+ * a step) is refused with HSV_ERR_LIMIT before any pass of that part runs. Every call on one GPU shares that GPU's array: calls that
  * overlap on one GPU (handles on different streams, or threads) race. Not
  * capturable into a HIP graph (it uploads the step data from host memory and
  * its renormalisation reads partials back to the host). */
