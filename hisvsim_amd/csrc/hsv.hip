@@ -65,6 +65,7 @@ struct hsv_state {
   int reg_waves = 4;                 // persistent-grid factor of the register part kernel
   int part_kernel = 2;               // 2: FP64 tile kernel for parts that fit a tile, else 1; 1: register-blocked kernel for k >= 8; 0: LDS kernel only
   int reg_bits = 0;                  // preferred register bits (0: per dtype)
+  int tile_bits = 0;                 // tile kernel width: 0 = 12 qubits (complex128) / 13 (complex64); one less = 256-thread tiles
   // Slots 0..low_slots-1 join every LDS-staged part when they fit: gathered
   // runs of 2^low_slots amplitudes (512 B for complex128 at 5) sustain
   // 5.0 TB/s on MI355X against 3.9 TB/s for 128 B runs.
@@ -447,17 +448,21 @@ int launch_reg_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t par
   return fail(HSV_ERR_LIMIT, "no register-kernel shape for a " + std::to_string(k) + "-qubit part");
 }
 
-// Tile width of the FP64 tile part kernel: 64 KB of LDS per workgroup.
-template <typename R>
-constexpr int tile_part_bits() { return sizeof(R) == 8 ? 12 : 13; }
+// Tile width of the FP64 tile part kernel: by default 64 KB of LDS per
+// workgroup of 512 threads (two workgroups per CU); tile_bits one less
+// gives 32 KB tiles of 256 threads (four per CU).
+int tile_part_bits(const hsv_state* s) {
+  const int def = s->dtype == HSV_C128 ? 12 : 13;
+  return s->tile_bits == def - 1 ? def - 1 : def;
+}
 
 // A part on the FP64 tile kernel (k_part_tile): the tile holds the part's
 // qubits padded with the lowest free slots (so slots 0-4 first) to 2^KT
 // amplitudes; the register plan is the register kernel's, over the tile's
-// bits, with KT - 9 register bits per thread.
-template <typename R>
+// bits, with KT - log2(NTH) register bits per thread.
+template <typename R, int KT, int NTH>
 int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t part_mask) {
-  constexpr int KT = tile_part_bits<R>();
+  constexpr int RB = KT - tile_log2(NTH);
   const int n = s->n;
   uint64_t tmask = part_mask;
   for (int b = 0; b < n && __builtin_popcountll(tmask) < KT; ++b) tmask |= 1ull << b;
@@ -468,7 +473,7 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
     if (tmask >> b & 1) slot[r++] = b;
   }
   RegPlan P;
-  plan_register_part(Ls, inner_of, KT, KT - 9, &P, 0);
+  plan_register_part(Ls, inner_of, KT, RB, &P, 0);
   // fast segments (only uncontrolled one-qubit dense gates and sign flips)
   // become rounds: the dense gates on distinct register slots, then a run of
   // sign flips; a dense gate after a flip, or on a slot the round already
@@ -476,7 +481,7 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
   // commute with)
   std::vector<TileRound> rounds;
   for (RegSeg& S : P.segs) {
-    if (__builtin_popcount(S.regmask) != KT - 9) return fail(HSV_ERR_LIMIT, "tile layout without its register bits");
+    if (__builtin_popcount(S.regmask) != RB) return fail(HSV_ERR_LIMIT, "tile layout without its register bits");
     S.fast = 1;
     for (int i = S.first; i < S.first + S.count; ++i) {
       const RegGate& G = P.gates[i];
@@ -501,7 +506,7 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
         // bits (r's bits on the set bits of regmask, lowest first) hold the rest
         const uint32_t rm = G.cmask & S.regmask;
         uint32_t rsel = 0;
-        for (uint32_t r = 0; r < (1u << (KT - 9)); ++r) {
+        for (uint32_t r = 0; r < (1u << RB); ++r) {
           uint32_t jr = 0, m = S.regmask;
           for (int b = 0; m; ++b, m &= m - 1)
             if (r >> b & 1) jr |= m & (~m + 1);
@@ -549,12 +554,19 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
   const uint64_t ntiles = 1ull << (n - KT);
   uint64_t top = 0;  // the highest tile_order outer slots vary fastest across workgroups
   for (int i = 0; i < s->bf.tile_order && (outer & ~top); ++i) top |= 1ull << (63 - __builtin_clzll(outer & ~top));
-  hipLaunchKernelGGL((k_part_tile<R, KT>), (unsigned)ntiles, 512, 0, s->stream, amp_as<R>(s),
+  hipLaunchKernelGGL((k_part_tile<R, KT, NTH>), (unsigned)ntiles, NTH, 0, s->stream, amp_as<R>(s),
                      reinterpret_cast<const TileTabs*>(dev), reinterpret_cast<const RegSeg*>(dev + seg_off),
                      (int)P.segs.size(), reinterpret_cast<const TileRound*>(dev + round_off),
                      reinterpret_cast<const RegGate*>(dev + gate_off), ntiles, outer, top);
   HIPCHK(hipGetLastError());
   return HSV_OK;
+}
+
+template <typename R>
+int launch_tile_part(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t part_mask) {
+  constexpr int KT = sizeof(R) == 8 ? 12 : 13;
+  return tile_part_bits(s) == KT ? launch_tile_part_t<R, KT, 512>(s, Ls, part_mask)
+                                 : launch_tile_part_t<R, KT - 1, 256>(s, Ls, part_mask);
 }
 
 template <typename R, int MODE>
@@ -876,7 +888,7 @@ int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_s
   s->basis0 = false;
   // part_kernel 2: every part whose qubits and slots 0-4 fit a tile takes
   // the FP64 tile kernel (it pads the tile with the lowest free slots itself)
-  const int kt = s->dtype == HSV_C128 ? tile_part_bits<double>() : tile_part_bits<float>();
+  const int kt = tile_part_bits(s);
   const uint64_t low5 = (1ull << std::min(s->n, 5)) - 1;
   const bool tile = s->part_kernel == 2 && s->n >= kt && __builtin_popcountll(part | low5) <= kt;
   const int kmax = max_part_qubits(s);
@@ -894,7 +906,7 @@ int hsv_apply_part(hsv_state* s, const hsv_gate* gates, int num_gates, int opt_s
   } else if (num_gates > 0) {
     const int k = __builtin_popcountll(part);
     const bool reg = !tile && s->part_kernel >= 1 && k >= 8 && k <= 13;
-    int rc = tile  ? (s->dtype == HSV_C128 ? launch_tile_part_t<double>(s, Ls, part) : launch_tile_part_t<float>(s, Ls, part))
+    int rc = tile  ? (s->dtype == HSV_C128 ? launch_tile_part<double>(s, Ls, part) : launch_tile_part<float>(s, Ls, part))
              : reg ? (s->dtype == HSV_C128 ? launch_reg_part_t<double>(s, Ls, part) : launch_reg_part_t<float>(s, Ls, part))
                    : (s->dtype == HSV_C128 ? launch_part_t<double>(s, Ls, part) : launch_part_t<float>(s, Ls, part));
     if (rc) return rc;
@@ -1334,6 +1346,7 @@ int hsv_tune(hsv_state* s, const char* key, int value) {
   if (k == "reg_waves" && value >= 1 && value <= 64) s->reg_waves = value;
   else if (k == "part_kernel" && value >= 0 && value <= 2) s->part_kernel = value;
   else if (k == "reg_bits" && value >= 0 && value <= 5) s->reg_bits = value;
+  else if (k == "tile_bits" && value >= 0 && value <= 13) s->tile_bits = value;
   else if (k == "low_slots" && value >= 0 && value <= 6) s->low_slots = value;
   else if (k == "io_bits" && value >= 0 && value <= 6) s->io_bits = value;
   else if (k == "fused_variant" && value >= 0 && value <= 2) s->bf.variant = value;

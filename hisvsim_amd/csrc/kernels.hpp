@@ -1021,56 +1021,20 @@ __device__ __forceinline__ uint32_t tile_thread_bits(uint32_t regmask) {
   return x;
 }
 
-// ---- FP64 tile part kernel ----------------------------------------------------
-// The part path's exact arithmetic (one FP64 sweep per gate, as the
-// reference's execute_on) on the tile layout of the BF16-split kernels: a
-// 2^KT-amplitude tile (the part's qubits, slots 0-4 and the lowest free
-// slots) is loaded with 512-byte runs per wave into LDS, the part's gates
-// run on it segment by segment in registers (each of the 512 threads holds
-// the 2^RB amplitudes of the segment's RB register bits, as in the register
-// kernel: reg_apply), and the tile is stored the same way. One workgroup
-// per tile, two per CU (64 KB of LDS each at KT = 12 in complex128), so one
-// workgroup's loads and stores overlap the other's gates.
-struct TileTabs {
-  uint64_t alo[64], ahi[256];   // address offset of tile bits 0-5 / 6-13
-};
+// The segments of a part on an LDS-resident tile of 2^KT amplitudes held by
+// NTH threads: per segment, each thread reads the 2^RB values of its
+// register layout (RB = KT - log2 NTH), applies the segment's gates (rounds
+// or gate records) and writes them back in place.
+__device__ __forceinline__ uint32_t tile_sw(uint32_t t) { return t ^ (((t >> 4) ^ (t >> 8)) & 15u); }
+constexpr int tile_log2(int x) { return x <= 1 ? 0 : 1 + tile_log2(x >> 1); }
 
-template <typename R, int KT>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_part_tile(
-    typename VecT<R>::type* __restrict__ a_vec, const TileTabs* __restrict__ tabs, const RegSeg* __restrict__ segs,
-    int nseg, const TileRound* __restrict__ rounds, const RegGate* __restrict__ gates, uint64_t ntiles,
-    uint64_t outer_mask, uint64_t top) {
+template <typename R, int KT, int NTH>
+__device__ __forceinline__ void tile_segments(cplx<R>* ys, const RegSeg* __restrict__ segs, int nseg,
+                                              const TileRound* __restrict__ rounds,
+                                              const RegGate* __restrict__ gates) {
   using V = cplx<R>;
-  using G = typename VecT<R>::type;
-  constexpr int NTH = 512, W = NTH / 64, T = 1 << KT, PER = T / NTH;
-  constexpr int RB = KT - 9, NR = 1 << RB;
-  static_assert(RB >= 1 && RB <= 5, "tile shape");
-  __shared__ V ys[T];
-  __shared__ uint64_t alo[64], ahi[T >> 6];
-  // LDS bank swizzle: tile element t at t ^ (((t >> 4) ^ (t >> 8)) & 15), so
-  // that the 16-byte elements a wave reads (its lanes spanning 6 tile bits)
-  // spread over the 16 four-bank groups for most choices of register bits
-  auto sw = [](uint32_t t) { return t ^ (((t >> 4) ^ (t >> 8)) & 15u); };
-  for (int i = threadIdx.x; i < 64; i += NTH) alo[i] = tabs->alo[i];
-  for (int i = threadIdx.x; i < (T >> 6); i += NTH) ahi[i] = tabs->ahi[i];
-  __syncthreads();
-  const uint64_t t0 = blockIdx.x;
-  if (t0 >= ntiles) return;  // uniform across the workgroup
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t base = tile_base(t0, outer_mask, top) + alo[lane];
-  {
-    // tile position t = x + NTH j: its lane bits select alo, the rest ahi
-    G raw[PER];
-#pragma unroll
-    for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a_vec[base + ahi[wave + W * j]]);
-#pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      V v;
-      v.x = raw[j].x;
-      v.y = raw[j].y;
-      ys[sw(threadIdx.x + NTH * j)] = v;
-    }
-  }
+  constexpr int RB = KT - tile_log2(NTH), NR = 1 << RB;
+  auto sw = tile_sw;
   for (int s = 0; s < nseg; ++s) {
     __syncthreads();  // the tile, or the previous segment's results, is in LDS
     const RegSeg S = segs[s];
@@ -1119,6 +1083,60 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
     for (int r = 0; r < NR; ++r) ys[swt ^ sw(jr[r])] = v[r];
   }
+}
+
+// ---- FP64 tile part kernel ----------------------------------------------------
+// The part path's exact arithmetic (one FP64 sweep per gate, as the
+// reference's execute_on) on the tile layout of the BF16-split kernels: a
+// 2^KT-amplitude tile (the part's qubits, slots 0-4 and the lowest free
+// slots) is loaded with 512-byte runs per wave into LDS, the part's gates
+// run on it segment by segment in registers (each of the NTH threads holds
+// the 2^RB amplitudes of the segment's RB = KT - log2 NTH register bits, as
+// in the register kernel: reg_apply), and the tile is stored the same way.
+// One workgroup per tile: two per CU at KT = 12, NTH = 512 in complex128
+// (64 KB of LDS each), four at KT = 11, NTH = 256.
+struct TileTabs {
+  uint64_t alo[64], ahi[256];   // address offset of tile bits 0-5 / 6-13
+};
+
+template <typename R, int KT, int NTH>
+__global__ __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(4))) void k_part_tile(
+    typename VecT<R>::type* __restrict__ a_vec, const TileTabs* __restrict__ tabs, const RegSeg* __restrict__ segs,
+    int nseg, const TileRound* __restrict__ rounds, const RegGate* __restrict__ gates, uint64_t ntiles,
+    uint64_t outer_mask, uint64_t top) {
+  using V = cplx<R>;
+  using G = typename VecT<R>::type;
+  constexpr int W = NTH / 64, T = 1 << KT, PER = T / NTH;
+  constexpr int RB = KT - tile_log2(NTH);
+  static_assert(RB >= 1 && RB <= 5 && (T >> 6) <= 256, "tile shape");
+  __shared__ V ys[T];
+  __shared__ uint64_t alo[64], ahi[T >> 6];
+  // LDS bank swizzle (tile_sw): tile element t at t ^ (((t >> 4) ^ (t >> 8))
+  // & 15), so that the 16-byte elements a wave reads (its lanes spanning 6
+  // tile bits) spread over the 16 four-bank groups for most choices of
+  // register bits
+  auto sw = tile_sw;
+  for (int i = threadIdx.x; i < 64; i += NTH) alo[i] = tabs->alo[i];
+  for (int i = threadIdx.x; i < (T >> 6); i += NTH) ahi[i] = tabs->ahi[i];
+  __syncthreads();
+  const uint64_t t0 = blockIdx.x;
+  if (t0 >= ntiles) return;  // uniform across the workgroup
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t base = tile_base(t0, outer_mask, top) + alo[lane];
+  {
+    // tile position t = x + NTH j: its lane bits select alo, the rest ahi
+    G raw[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a_vec[base + ahi[wave + W * j]]);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      V v;
+      v.x = raw[j].x;
+      v.y = raw[j].y;
+      ys[sw(threadIdx.x + NTH * j)] = v;
+    }
+  }
+  tile_segments<R, KT, NTH>(ys, segs, nseg, rounds, gates);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < PER; ++j) {

@@ -741,6 +741,45 @@ def test_tile_part_kernel_layered_fast_segments(orc, seed):
     assert np.abs(got - want).max() <= TOL * max(1.0, np.abs(want).max())
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+@pytest.mark.parametrize("n", [14, 21])
+def test_tile_part_kernel_256_threads(orc, dtype, n):
+    """tile_bits one below the default: 11-qubit tiles of 256 threads for
+    complex128 (12 for complex64; 32 KB of LDS, four workgroups per CU) on
+    layered circuits with extra CCZ and controlled U3 gates (rounds and the
+    general gate loop), parts of up to that width with slots 0-4, against
+    the oracle at 1e-12 (complex64 1e-5)."""
+    rng = np.random.default_rng(n)
+    gates = []
+    for g in random_layered(n, 6, seed=n):
+        gates.append(g)
+        u = rng.random()
+        if u < 0.04:
+            q = [int(x) for x in rng.choice(n, 3, replace=False)]
+            gates.append(Gate("Z", (q[0],), (q[1], q[2])))
+        elif u < 0.08:
+            q = [int(x) for x in rng.choice(n, 2, replace=False)]
+            gates.append(Gate("U3", (q[0],), (q[1],), (0.3, 0.7, 1.1)))
+    from hisvsim_amd.partition import closure_parts
+    kt = 11 if dtype == "complex128" else 12
+    parts = closure_parts(gates, kt, always=range(5))
+    a = rand_state(n, 4)
+    if dtype == "complex64":
+        a = a.astype(np.complex64).astype(complex)
+    print(f"[tile_bits] n={n} {dtype} kt={kt} parts={len(parts)}", flush=True)
+    want = a.copy()
+    orc.apply_gates(want, gates)
+    with capi.State(n, dtype) as s:
+        s.tune("part_kernel", 2)
+        s.tune("tile_bits", kt)
+        s.write(a)
+        for p in parts:
+            s.apply_part(p, 0)
+        got = s.read()
+    tol = TOL if dtype == "complex128" else 1e-5
+    assert np.abs(got - want).max() <= tol * max(1.0, np.abs(want).max())
+
+
 @pytest.mark.parametrize("kernel", [1, 2])
 def test_tile_and_register_kernels_agree_on_layered(kernel):
     """The tile and register part kernels against the LDS kernel on a
