@@ -10,11 +10,13 @@
 //  * init: every amplitude = 2^(-n/2) (the state after the H layer);
 //  * stage A, per layer: contiguous chunks of 2^L amplitudes (the L low
 //    qubits) in LDS: multiply by the layer's diagonal
-//    prod_edges ZZ = exp(-i gamma/2 (|E| - 2 cut(x))), looked up by cut(x),
-//    then RX(2 beta) on each low qubit;
+//    prod_edges ZZ = exp(-i gamma/2 (|E| - 2 cut(x))), looked up by cut(x)
+//    (cut(x) itself from a per-instance table over the low qubits and two
+//    per-chunk tables for the edges that reach the chunk's fixed high bits),
+//    then RX(2 beta) on each low qubit, three qubits per LDS sweep in registers;
 //  * stage B, per layer, instances with n > L: chunks over the high qubits
 //    plus the 3 lowest (contiguous 64 B runs), RX(2 beta) on each high qubit;
-//  * reduce: sum_x |a_x|^2 cut(x) per chunk in FP64, then per instance.
+//  * reduce: sum_x |a_x|^2 cut(x) per stage-A chunk in FP64, then per instance.
 // The stage kernels are persistent: a workgroup walks the flattened list of
 // (instance, chunk) items of its stage. The whole sequence (plus the copy of
 // the results to pinned host memory) is captured once into a hipGraph and
@@ -69,27 +71,118 @@ struct Stage {
 
 struct cf { float x, y; };
 
-__device__ __forceinline__ int cut_of(uint64_t x, const unsigned char* eu, const unsigned char* ev, int ne) {
-  int c = 0;
-  for (int e = 0; e < ne; ++e) c += (int)(((x >> eu[e]) ^ (x >> ev[e])) & 1u);
-  return c;
+// Cut sizes of a chunk x = h + l (h: the fixed bits from B up, l < 2^B, B <= 13)
+// from tables: cut(x) = low[l] + T1[l & 63] + T2[l >> 6], where low[] (host,
+// per instance) counts the cut edges with both ends below B, and the chunk's
+// tables hold the rest: an edge with both ends at or above B adds the
+// constant h_u ^ h_v; an edge from a low qubit a to a high qubit c adds
+// l_a ^ h_c = h_c + l_a (1 - 2 h_c), a constant plus a weight d_a on bit a.
+// T1 sums the weights of bits 0-5 of its index, T2 those of bits 6-12 plus
+// the constant. All threads of the workgroup take part (two barriers).
+template <int NT>
+__device__ __forceinline__ void chunk_cut_tables(uint64_t h, int B, const unsigned char* su, const unsigned char* sv,
+                                                 int ne, int* sd, short* T1, short* T2) {
+  for (int t = threadIdx.x; t <= B; t += NT) {
+    // t < B: weight d_t of low bit t; t == B: the constant
+    int acc = 0;
+    for (int e = 0; e < ne; ++e) {
+      int a = su[e], c = sv[e];
+      if (a >= B) { const int x = a; a = c; c = x; }   // a: the lower end if any end is low
+      const int hc = (int)((h >> c) & 1u);
+      if (a < B && c >= B) {
+        if (t == a) acc += 1 - 2 * hc;
+        else if (t == B) acc += hc;
+      } else if (a >= B && t == B) {
+        acc += (int)(((h >> a) ^ (h >> c)) & 1u);
+      }
+    }
+    sd[t] = acc;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 64 + 128; t += NT) {
+    int acc = 0;
+    if (t < 64) {
+      for (int u = 0; u < 6 && u < B; ++u)
+        if (t >> u & 1) acc += sd[u];
+      T1[t] = (short)acc;
+    } else {
+      const int i = t - 64;
+      acc = sd[B];
+      for (int u = 6; u < B; ++u)
+        if (i >> (u - 6) & 1) acc += sd[u];
+      T2[i] = (short)acc;
+    }
+  }
+  __syncthreads();
 }
 
-// RX(2 beta) = [[cos b, -i sin b], [-i sin b, cos b]] on LDS bit q of a 2^bits block.
+// LDS position of chunk element i: bits 0-2 XOR bits 5-7 and bits 3-4 XOR
+// bits 6-7, so that the 64 lanes of a wave reading 8-byte elements spread
+// over all bank pairs, for consecutive elements as for the three-bit
+// register groups of rx_block (a bijection on [0, 2^bits) for any bits).
+__device__ __forceinline__ uint32_t qsw(uint32_t i) { return i ^ ((i >> 5) & 7u) ^ (((i >> 6) & 3u) << 3); }
+
+// RX(2 beta) = [[cos b, -i sin b], [-i sin b, cos b]] on a pair.
+__device__ __forceinline__ void rx_pair(cf& a, cf& b, float c, float sn) {
+  const cf x = a, y = b;
+  a.x = c * x.x + sn * y.y;   // c a - i s b
+  a.y = c * x.y - sn * y.x;
+  b.x = sn * x.y + c * y.x;   // -i s a + c b
+  b.y = -sn * x.x + c * y.y;
+}
+
+// RX on LDS bit q of a 2^bits block, one pair per thread and step.
 __device__ __forceinline__ void rx_lds(cf* s, int bits, int q, float c, float sn, int nt) {
   const uint32_t half = 1u << (bits - 1);
   for (uint32_t w = threadIdx.x; w < half; w += nt) {
     const uint32_t lo = w & ((1u << q) - 1);
     const uint32_t i0 = ((w ^ lo) << 1) | lo;
     const uint32_t i1 = i0 | (1u << q);
-    const cf a = s[i0], b = s[i1];
-    cf o0, o1;
-    o0.x = c * a.x + sn * b.y;   // c a - i s b
-    o0.y = c * a.y - sn * b.x;
-    o1.x = sn * a.y + c * b.x;   // -i s a + c b
-    o1.y = -sn * a.x + c * b.y;
-    s[i0] = o0;
-    s[i1] = o1;
+    cf a = s[qsw(i0)], b = s[qsw(i1)];
+    rx_pair(a, b, c, sn);
+    s[qsw(i0)] = a;
+    s[qsw(i1)] = b;
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void rx_regs(cf* v, float c, float sn) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+    if (!(r & S)) rx_pair(v[r], v[r + S], c, sn);
+}
+
+// RX on bits [from, bits) of a 2^bits LDS block, three bits per sweep: each
+// thread takes the 8 elements of three consecutive bits q0..q0+2 into
+// registers, applies RX on those of them not done yet and writes them back;
+// one barrier per sweep (the block must be in LDS, after a barrier, on entry).
+template <int NT>
+__device__ __forceinline__ void rx_block(cf* s, int bits, int from, float c, float sn) {
+  if (bits < 3) {
+    for (int q = from; q < bits; ++q) {
+      rx_lds(s, bits, q, c, sn, NT);
+      __syncthreads();
+    }
+    return;
+  }
+  for (int done = from; done < bits;) {
+    const int q0 = done < bits - 3 ? done : bits - 3;
+    const int first = done - q0;  // 0..2: register bits below it are done
+    const uint32_t nw = 1u << (bits - 3);
+    for (uint32_t w = threadIdx.x; w < nw; w += NT) {
+      const uint32_t lo = w & ((1u << q0) - 1);
+      const uint32_t base = ((w ^ lo) << 3) | lo;
+      cf v[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = s[qsw(base + ((uint32_t)r << q0))];
+      if (first == 0) rx_regs<1>(v, c, sn);
+      if (first <= 1) rx_regs<2>(v, c, sn);
+      rx_regs<4>(v, c, sn);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) s[qsw(base + ((uint32_t)r << q0))] = v[r];
+    }
+    __syncthreads();
+    done = q0 + 3;
   }
 }
 
@@ -124,11 +217,15 @@ __global__ __launch_bounds__(NT) void k_qaoa_low(cf* __restrict__ st, const Inst
                                                  int num, uint64_t items, int L, int layer,
                                                  const unsigned char* __restrict__ eu,
                                                  const unsigned char* __restrict__ ev,
+                                                 const unsigned char* __restrict__ cut_low,
+                                                 const uint64_t* __restrict__ cut_off,
                                                  const double* __restrict__ gamma, const double* __restrict__ beta) {
   extern __shared__ __align__(16) unsigned char smem[];
   cf* s = reinterpret_cast<cf*>(smem);
   __shared__ float ph_x[kMaxEdges + 1], ph_y[kMaxEdges + 1];
   __shared__ unsigned char su[kMaxEdges], sv[kMaxEdges];
+  __shared__ int sd[14];
+  __shared__ short T1[64], T2[128];
   int cur = -1;
   for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
     const int k = find_inst(cum, num, it);
@@ -152,23 +249,22 @@ __global__ __launch_bounds__(NT) void k_qaoa_low(cf* __restrict__ st, const Inst
       }
       __syncthreads();
     }
+    chunk_cut_tables<NT>(gbase, bits, su, sv, I.ne, sd, T1, T2);
+    const unsigned char* low = cut_low + cut_off[id];
     const uint32_t sz = 1u << bits;
     for (uint32_t j = threadIdx.x; j < sz; j += NT) {
       const cf a = st[base + j];
-      const int c = cut_of(gbase + j, su, sv, I.ne);
+      const int c = (int)low[j] + T1[j & 63] + T2[j >> 6];
       cf o;
       o.x = a.x * ph_x[c] - a.y * ph_y[c];
       o.y = a.x * ph_y[c] + a.y * ph_x[c];
-      s[j] = o;
+      s[qsw(j)] = o;
     }
     __syncthreads();
     const double b = beta[I.angle_off + layer];
     const float cb = (float)std::cos(b), sb = (float)std::sin(b);
-    for (int q = 0; q < bits; ++q) {
-      rx_lds(s, bits, q, cb, sb, NT);
-      __syncthreads();
-    }
-    for (uint32_t j = threadIdx.x; j < sz; j += NT) st[base + j] = s[j];
+    rx_block<NT>(s, bits, 0, cb, sb);
+    for (uint32_t j = threadIdx.x; j < sz; j += NT) st[base + j] = s[qsw(j)];
   }
 }
 
@@ -202,29 +298,31 @@ __global__ __launch_bounds__(NT) void k_qaoa_high(cf* __restrict__ st, const Ins
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < sz; j += NT) {
       const uint64_t idx = base + (j & ((1u << kLowContig) - 1)) + ((uint64_t)(j >> kLowContig) << L);
-      s[j] = st[I.off + idx];
+      s[qsw(j)] = st[I.off + idx];
     }
     __syncthreads();
     const double b = beta[I.angle_off + layer];
     const float cb = (float)std::cos(b), sb = (float)std::sin(b);
-    for (int q = kLowContig; q < bits; ++q) {
-      rx_lds(s, bits, q, cb, sb, NT);
-      __syncthreads();
-    }
+    rx_block<NT>(s, bits, kLowContig, cb, sb);
     for (uint32_t j = threadIdx.x; j < sz; j += NT) {
       const uint64_t idx = base + (j & ((1u << kLowContig) - 1)) + ((uint64_t)(j >> kLowContig) << L);
-      st[I.off + idx] = s[j];
+      st[I.off + idx] = s[qsw(j)];
     }
   }
 }
 
-// sum |a|^2 cut(x) over chunks of 4096 amplitudes, FP64, one partial per item
+// sum |a|^2 cut(x) over the chunks of stage A (2^min(n, L) amplitudes), FP64,
+// one partial per item
 __global__ __launch_bounds__(256) void k_qaoa_cost(const cf* __restrict__ st, const Inst* __restrict__ inst,
                                                    const int* __restrict__ ids, const uint64_t* __restrict__ cum,
-                                                   int num, uint64_t items, const unsigned char* __restrict__ eu,
-                                                   const unsigned char* __restrict__ ev, double* __restrict__ part) {
+                                                   int num, uint64_t items, int L, const unsigned char* __restrict__ eu,
+                                                   const unsigned char* __restrict__ ev,
+                                                   const unsigned char* __restrict__ cut_low,
+                                                   const uint64_t* __restrict__ cut_off, double* __restrict__ part) {
   __shared__ double red[256];
   __shared__ unsigned char su[kMaxEdges], sv[kMaxEdges];
+  __shared__ int sd[14];
+  __shared__ short T1[64], T2[128];
   int cur = -1;
   for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
     const int k = find_inst(cum, num, it);
@@ -239,12 +337,14 @@ __global__ __launch_bounds__(256) void k_qaoa_cost(const cf* __restrict__ st, co
       }
       __syncthreads();
     }
-    const uint64_t n_amp = 1ull << I.n;
-    const uint64_t lo = (it - cum[k]) * 4096;
+    const int bits = I.n < L ? I.n : L;
+    const uint64_t lo = (it - cum[k]) << bits;
+    chunk_cut_tables<256>(lo, bits, su, sv, I.ne, sd, T1, T2);
+    const unsigned char* low = cut_low + cut_off[id];
     double acc = 0.0;
-    for (uint64_t j = lo + threadIdx.x; j < lo + 4096 && j < n_amp; j += 256) {
-      const cf a = st[I.off + j];
-      acc += ((double)a.x * a.x + (double)a.y * a.y) * cut_of(j, su, sv, I.ne);
+    for (uint32_t j = threadIdx.x; j < (1u << bits); j += 256) {
+      const cf a = st[I.off + lo + j];
+      acc += ((double)a.x * a.x + (double)a.y * a.y) * (int)(low[j] + T1[j & 63] + T2[j >> 6]);
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -288,6 +388,8 @@ struct hsv_qaoa_batch {
   double *d_gamma = nullptr, *d_beta = nullptr;
   // stage work lists (device): ids and cum for init, low(l), high(l), cost
   void* d_lists = nullptr;
+  unsigned char* d_cut = nullptr;    // per instance: cut sizes of the edges among its low min(n, L) qubits
+  uint64_t* d_cut_off = nullptr;     // per instance: offset of its table in d_cut
   double* d_part = nullptr;
   double* d_out = nullptr;
   double* h_out = nullptr;  // pinned
@@ -325,15 +427,19 @@ void destroy_graph(hsv_qaoa_batch* b) {
   b->graph = nullptr;
   if (b->d_lists) (void)hipFree(b->d_lists);
   if (b->d_part) (void)hipFree(b->d_part);
+  if (b->d_cut) (void)hipFree(b->d_cut);
+  if (b->d_cut_off) (void)hipFree(b->d_cut_off);
   b->d_lists = nullptr;
   b->d_part = nullptr;
+  b->d_cut = nullptr;
+  b->d_cut_off = nullptr;
 }
 
 template <int NT>
 void launch_low(hsv_qaoa_batch* b, int grid, const int* ids, const uint64_t* cum, int num, uint64_t items, int layer) {
   const size_t lds = (size_t(1) << b->L) * sizeof(cf);
   hipLaunchKernelGGL((k_qaoa_low<NT>), grid, NT, lds, b->stream, b->state, b->d_inst, ids, cum, num, items, b->L,
-                     layer, b->d_eu, b->d_ev, b->d_gamma, b->d_beta);
+                     layer, b->d_eu, b->d_ev, b->d_cut, b->d_cut_off, b->d_gamma, b->d_beta);
 }
 template <int NT>
 void launch_high(hsv_qaoa_batch* b, int grid, const int* ids, const uint64_t* cum, int num, uint64_t items, int layer,
@@ -357,7 +463,31 @@ int build_graph(hsv_qaoa_batch* b) {
       return (I.p > l && I.n > L) ? (1ull << I.n) >> (kLowContig + I.n - L) : 0;
     }));
   }
-  lists.push_back(make_list(b, [](const Inst& I) { return ((1ull << I.n) + 4095) / 4096; }));  // cost (all)
+  lists.push_back(make_list(b, [&](const Inst& I) { return (1ull << I.n) >> std::min(I.n, L); }));  // cost (all)
+  {
+    // cut sizes over each instance's low min(n, L) qubits, from the edges
+    // with both ends there (the kernels add the other edges per chunk)
+    std::vector<uint64_t> coff(b->num);
+    uint64_t tot = 0;
+    for (int i = 0; i < b->num; ++i) {
+      coff[i] = tot;
+      tot += 1ull << std::min(b->inst[i].n, L);
+    }
+    std::vector<unsigned char> cut(tot, 0);
+    for (int i = 0; i < b->num; ++i) {
+      const Inst& I = b->inst[i];
+      const int B = std::min(I.n, L);
+      for (int e = 0; e < I.ne; ++e) {
+        const int u = b->eu[I.edge_off + e], v = b->ev[I.edge_off + e];
+        if (u >= B || v >= B) continue;
+        for (uint64_t x = 0; x < (1ull << B); ++x) cut[coff[i] + x] += (unsigned char)(((x >> u) ^ (x >> v)) & 1u);
+      }
+    }
+    QCHK(hipMalloc(reinterpret_cast<void**>(&b->d_cut), std::max<uint64_t>(tot, 1)));
+    QCHK(hipMalloc(reinterpret_cast<void**>(&b->d_cut_off), sizeof(uint64_t) * b->num));
+    QCHK(hipMemcpy(b->d_cut, cut.data(), tot, hipMemcpyHostToDevice));
+    QCHK(hipMemcpy(b->d_cut_off, coff.data(), sizeof(uint64_t) * b->num, hipMemcpyHostToDevice));
+  }
   // device layout per list: ids (int, padded to 8 B) | cum (u64)
   std::vector<size_t> offs;
   size_t bytes = 0;
@@ -414,7 +544,8 @@ int build_graph(hsv_qaoa_batch* b) {
     const HostList& C = lists[ic];
     if (C.items) {
       hipLaunchKernelGGL(k_qaoa_cost, (int)std::min<uint64_t>(C.items, grid), 256, 0, b->stream, b->state, b->d_inst,
-                         ids_of(ic), cum_of(ic), (int)C.ids.size(), C.items, b->d_eu, b->d_ev, b->d_part);
+                         ids_of(ic), cum_of(ic), (int)C.ids.size(), C.items, L, b->d_eu, b->d_ev, b->d_cut,
+                         b->d_cut_off, b->d_part);
       hipLaunchKernelGGL(k_qaoa_sum, (b->num + 255) / 256, 256, 0, b->stream, b->d_part, cum_of(ic),
                          (int)C.ids.size(), C.items, b->d_out);
     }
