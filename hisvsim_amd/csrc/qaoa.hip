@@ -15,7 +15,8 @@
 //    per-chunk tables for the edges that reach the chunk's fixed high bits),
 //    then RX(2 beta) on each low qubit, three qubits per LDS sweep in registers;
 //  * stage B, per layer, instances with n > L: chunks over the high qubits
-//    plus the 3 lowest (contiguous 64 B runs), RX(2 beta) on each high qubit;
+//    plus the 3 to 6 lowest (contiguous 64 to 512 B runs, as many as a
+//    2^13-amplitude chunk leaves room for), RX(2 beta) on each high qubit;
 //  * reduce: sum_x |a_x|^2 cut(x) per stage-A chunk in FP64, then per instance.
 // The stage kernels are persistent: a workgroup walks the flattened list of
 // (instance, chunk) items of its stage. The whole sequence (plus the copy of
@@ -49,7 +50,16 @@ int qfail(int code, const std::string& m) {
 
 constexpr int kMaxEdges = 64;
 constexpr int kMaxQaoaQubits = 26;
-constexpr int kLowContig = 3;  // low qubits carried along in stage B
+// Stage B chunks hold an instance's hb = n - L high qubits and its lowest
+// low_contig(hb) qubits (contiguous runs of 2^that amplitudes), at most
+// kStageBBits qubits in all: from kLowContig (64-byte runs) up to
+// kMaxLowContig (512-byte runs) for the narrower instances.
+constexpr int kLowContig = 3;
+constexpr int kMaxLowContig = 6;
+constexpr int kStageBBits = 13;
+__host__ __device__ inline int low_contig(int hb) {
+  return kStageBBits - hb < kMaxLowContig ? kStageBBits - hb : kMaxLowContig;
+}
 
 struct Inst {
   uint64_t off;      // first amplitude in the batch buffer
@@ -195,19 +205,21 @@ __device__ __forceinline__ int find_inst(const uint64_t* cum, int num, uint64_t 
   return lo;
 }
 
-__global__ void k_qaoa_init(cf* __restrict__ st, const Inst* __restrict__ inst, const int* __restrict__ ids,
-                            const uint64_t* __restrict__ cum, int num, uint64_t items) {
-  // item = 256 consecutive amplitudes
+constexpr int kInitItem = 4096;  // amplitudes per item of k_qaoa_init
+
+__global__ __launch_bounds__(256) void k_qaoa_init(cf* __restrict__ st, const Inst* __restrict__ inst,
+                                                   const int* __restrict__ ids, const uint64_t* __restrict__ cum,
+                                                   int num, uint64_t items) {
+  // item = kInitItem consecutive amplitudes (fewer for instances below 12 qubits)
   for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
     const int k = find_inst(cum, num, it);
     const Inst I = inst[ids[k]];
-    const uint64_t lo = (it - cum[k]) * 256 + threadIdx.x;
-    if (lo < (1ull << I.n)) {
-      cf v;
-      v.x = (float)(1.0 / std::sqrt((double)(1ull << I.n)));
-      v.y = 0.0f;
-      st[I.off + lo] = v;
-    }
+    const uint64_t n_amp = 1ull << I.n;
+    const uint64_t lo = (it - cum[k]) * kInitItem;
+    cf v;
+    v.x = (float)(1.0 / std::sqrt((double)n_amp));
+    v.y = 0.0f;
+    for (uint64_t j = lo + threadIdx.x; j < lo + kInitItem && j < n_amp; j += 256) st[I.off + j] = v;
   }
 }
 
@@ -288,24 +300,25 @@ __global__ __launch_bounds__(NT) void k_qaoa_high(cf* __restrict__ st, const Ins
   for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
     const int k = find_inst(cum, num, it);
     const Inst I = inst[ids[k]];
-    // chunk bits: the low kLowContig qubits + the high qubits L..n-1
+    // chunk bits: the lowest lc qubits + the high qubits L..n-1
     const int hb = I.n - L;
-    const int bits = kLowContig + hb;
-    const uint64_t cmask = ((1ull << kLowContig) - 1) | (((1ull << hb) - 1) << L);
+    const int lc = low_contig(hb);
+    const int bits = lc + hb;
+    const uint64_t cmask = ((1ull << lc) - 1) | (((1ull << hb) - 1) << L);
     const uint64_t omask = ((1ull << I.n) - 1) & ~cmask;
     const uint64_t base = pdep_q(it - cum[k], omask);
     const uint32_t sz = 1u << bits;
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < sz; j += NT) {
-      const uint64_t idx = base + (j & ((1u << kLowContig) - 1)) + ((uint64_t)(j >> kLowContig) << L);
+      const uint64_t idx = base + (j & ((1u << lc) - 1)) + ((uint64_t)(j >> lc) << L);
       s[qsw(j)] = st[I.off + idx];
     }
     __syncthreads();
     const double b = beta[I.angle_off + layer];
     const float cb = (float)std::cos(b), sb = (float)std::sin(b);
-    rx_block<NT>(s, bits, kLowContig, cb, sb);
+    rx_block<NT>(s, bits, lc, cb, sb);
     for (uint32_t j = threadIdx.x; j < sz; j += NT) {
-      const uint64_t idx = base + (j & ((1u << kLowContig) - 1)) + ((uint64_t)(j >> kLowContig) << L);
+      const uint64_t idx = base + (j & ((1u << lc) - 1)) + ((uint64_t)(j >> lc) << L);
       st[I.off + idx] = s[qsw(j)];
     }
   }
@@ -454,13 +467,13 @@ int build_graph(hsv_qaoa_batch* b) {
   destroy_graph(b);
   const int L = b->L;
   std::vector<HostList> lists;
-  lists.push_back(make_list(b, [](const Inst& I) { return ((1ull << I.n) + 255) / 256; }));  // init
+  lists.push_back(make_list(b, [](const Inst& I) { return ((1ull << I.n) + kInitItem - 1) / kInitItem; }));  // init
   for (int l = 0; l < b->pmax; ++l) {
     lists.push_back(make_list(b, [&](const Inst& I) -> uint64_t {
       return I.p > l ? (1ull << I.n) >> std::min(I.n, L) : 0;
     }));
     lists.push_back(make_list(b, [&](const Inst& I) -> uint64_t {
-      return (I.p > l && I.n > L) ? (1ull << I.n) >> (kLowContig + I.n - L) : 0;
+      return (I.p > l && I.n > L) ? (1ull << I.n) >> (low_contig(I.n - L) + I.n - L) : 0;
     }));
   }
   lists.push_back(make_list(b, [&](const Inst& I) { return (1ull << I.n) >> std::min(I.n, L); }));  // cost (all)
@@ -514,15 +527,15 @@ int build_graph(hsv_qaoa_batch* b) {
   };
   int maxhigh = 0;
   for (const Inst& I : b->inst)
-    if (I.n > L) maxhigh = std::max(maxhigh, kLowContig + I.n - L);
+    if (I.n > L) maxhigh = std::max(maxhigh, low_contig(I.n - L) + I.n - L);
   const int grid = b->num_cu * b->grid_per_cu;
 
   QCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   {
     const HostList& h = lists[0];
     if (h.items)
-      hipLaunchKernelGGL(k_qaoa_init, (int)std::min<uint64_t>(h.items, 65536), 256, 0, b->stream, b->state, b->d_inst,
-                         ids_of(0), cum_of(0), (int)h.ids.size(), h.items);
+      hipLaunchKernelGGL(k_qaoa_init, (int)std::min<uint64_t>(h.items, (uint64_t)b->num_cu * 8), 256, 0, b->stream,
+                         b->state, b->d_inst, ids_of(0), cum_of(0), (int)h.ids.size(), h.items);
     for (int l = 0; l < b->pmax; ++l) {
       const size_t ia = 1 + 2 * l, ib = 2 + 2 * l;
       const HostList& A = lists[ia];
@@ -655,7 +668,7 @@ int hsv_qaoa_tune(hsv_qaoa_batch* b, const char* key, int value) {
   // stage B holds the high qubits plus the low ones; keep it within 64 KB of LDS
   int maxn = 0;
   for (const Inst& I : b->inst) maxn = std::max(maxn, I.n);
-  if (kLowContig + maxn - b->L > 13) return qfail(HSV_ERR_LIMIT, "chunk_bits too small for the widest instance");
+  if (kLowContig + maxn - b->L > kStageBBits) return qfail(HSV_ERR_LIMIT, "chunk_bits too small for the widest instance");
   destroy_graph(b);
   return HSV_OK;
 }
@@ -673,7 +686,7 @@ int hsv_qaoa_run(hsv_qaoa_batch* b, double* expectations) {
   if (!b || !expectations) return qfail(HSV_ERR_ARG, "null argument");
   int maxn = 0;
   for (const Inst& I : b->inst) maxn = std::max(maxn, I.n);
-  if (kLowContig + maxn - b->L > 13) return qfail(HSV_ERR_LIMIT, "chunk_bits too small for the widest instance");
+  if (kLowContig + maxn - b->L > kStageBBits) return qfail(HSV_ERR_LIMIT, "chunk_bits too small for the widest instance");
   if (!b->exec)
     if (int rc = build_graph(b)) return rc;
   QCHK(hipEventRecord(b->e0, b->stream));
