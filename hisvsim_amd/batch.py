@@ -159,6 +159,13 @@ def apply_tuning(batch: CircuitBatch) -> None:
                 pass
 
 
+def _saved_shape():
+    if os.path.exists(TUNING_FILE):
+        with open(TUNING_FILE) as f:
+            return json.load(f).get("best")
+    return None
+
+
 def main(argv=None) -> int:
     import argparse
     ap = argparse.ArgumentParser(description="batched small circuits")
@@ -196,7 +203,7 @@ def main(argv=None) -> int:
                             "bytes_per_batch": nbytes,
                             "per_batch": f"2 x {S} B x the amplitudes each stage moves (every segment reads and "
                                          f"writes its instance once)"},
-               "tuning": (tuning or {}).get("best"), "dtype": args.dtype, "reps": args.reps}
+               "tuning": (tuning or {}).get("best") or _saved_shape(), "dtype": args.dtype, "reps": args.reps}
     print(json.dumps(out), flush=True)
     return 0
 

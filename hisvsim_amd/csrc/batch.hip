@@ -12,8 +12,9 @@
 // segment of every instance that has one: a persistent kernel walks the
 // flattened (instance, chunk) list of the stage, gathers each chunk (the
 // segment's qubits plus the instance's lowest other qubits) into LDS,
-// applies the segment's gates there with the LDS part kernel's gate sweeps,
-// and scatters it back. The stage launches (after a fill kernel that sets
+// applies the segment's gates there and scatters it back. A chunk of 12
+// qubits runs the FP64 tile kernel's plan (gates in registers, one LDS
+// exchange per register layout); smaller ones sweep LDS once per gate. The stage launches (after a fill kernel that sets
 // every instance to |0...0>) are captured once into a hipGraph; gate
 // matrices live in device memory, so new parameters for the same gate
 // structure replay the same graph.
@@ -27,6 +28,7 @@
 #include "../../include/hsv.h"
 #include "gates.hpp"
 #include "kernels.hpp"
+#include "tileplan.hpp"
 
 void hsv_set_last_error(const std::string& msg);
 
@@ -55,8 +57,15 @@ struct BSeg {
   int bits;             // popcount(chunk_mask)
   int gate_first;       // first PartGate record
   int gate_count;
+  int tseg_first;       // chunks of kTileBits qubits: first register segment of its tile plan
+  int tseg_count;       // -1: the gates sweep the chunk in LDS one by one
   int pad;
 };
+
+// Chunks of this many qubits run the FP64 tile kernel's register segments
+// (tile_segments: each thread holds 2^(kTileBits - log2 threads) amplitudes,
+// one LDS exchange per segment instead of one LDS sweep per gate).
+constexpr int kTileBits = 12;
 
 // Stage work list: the segments taking part (seg_ids) and the first
 // flattened item of each (cum).
@@ -103,7 +112,10 @@ template <typename R, int BS>
 __global__ __launch_bounds__(BS) void k_batch_stage(typename VecT<R>::type* __restrict__ st,
                                                     const BSeg* __restrict__ segs, const int* __restrict__ ids,
                                                     const uint64_t* __restrict__ cum, int num, uint64_t items,
-                                                    const PartGate* __restrict__ gates) {
+                                                    const PartGate* __restrict__ gates,
+                                                    const RegSeg* __restrict__ tsegs,
+                                                    const TileRound* __restrict__ trounds,
+                                                    const RegGate* __restrict__ tgates) {
   using V = typename VecT<R>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   V* lds = reinterpret_cast<V*>(smem_raw);
@@ -125,6 +137,28 @@ __global__ __launch_bounds__(BS) void k_batch_stage(typename VecT<R>::type* __re
     }
     const uint64_t base = S.amp_off + pdep64(it - cum[k], S.outer_mask);
     const uint32_t nin = 1u << S.bits;
+    if constexpr (BS >= 128) {
+      if (S.tseg_count >= 0) {  // uniform: a 2^kTileBits chunk with a tile plan
+        cplx<R>* ys = reinterpret_cast<cplx<R>*>(smem_raw);
+        for (uint32_t j = threadIdx.x; j < nin; j += BS) {
+          const V a = st[base + tlo[j & 63] + thi[j >> 6]];
+          cplx<R> c;
+          c.x = a.x;
+          c.y = a.y;
+          ys[tile_sw(j)] = c;
+        }
+        tile_segments<R, kTileBits, BS>(ys, tsegs + S.tseg_first, S.tseg_count, trounds, tgates);
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < nin; j += BS) {
+          const cplx<R> c = ys[tile_sw(j)];
+          V o;
+          o.x = c.x;
+          o.y = c.y;
+          st[base + tlo[j & 63] + thi[j >> 6]] = o;
+        }
+        continue;
+      }
+    }
     for (uint32_t j = threadIdx.x; j < nin; j += BS) lds[j] = st[base + tlo[j & 63] + thi[j >> 6]];
     __syncthreads();
     for (int g = 0; g < S.gate_count; ++g) {
@@ -235,6 +269,11 @@ struct hsv_batch {
   std::vector<StageList> stages;
   BSeg* d_segs = nullptr;
   PartGate* d_gates = nullptr;
+  // tile plans of the kTileBits-qubit segments, one device buffer:
+  // register segments | rounds | gate records | dense two-qubit matrices
+  void* d_tile = nullptr;
+  size_t tile_bytes = 0;
+  size_t tile_nseg = 0, tile_nround = 0, tile_ngate = 0, tile_nmat = 0;
   void* d_lists = nullptr;
   std::vector<size_t> list_off;
   uint64_t* d_starts = nullptr;
@@ -262,9 +301,12 @@ void drop_plan(hsv_batch* b) {
   (void)hipFree(b->d_segs);
   (void)hipFree(b->d_gates);
   (void)hipFree(b->d_lists);
+  (void)hipFree(b->d_tile);
   b->d_segs = nullptr;
   b->d_gates = nullptr;
   b->d_lists = nullptr;
+  b->d_tile = nullptr;
+  b->tile_bytes = 0;
   b->segs.clear();
   b->rec_gate.clear();
   b->rec_inner.clear();
@@ -353,6 +395,87 @@ int plan(hsv_batch* b, std::vector<PartGate>* recs) {
   return HSV_OK;
 }
 
+// The tile plans of every segment whose chunk holds kTileBits qubits (with
+// at least 128 threads per workgroup: at most 5 register bits), as one set
+// of arrays with absolute indices; sets each segment's tseg_first/count
+// (-1: LDS sweeps). all[g]: gate g of the batch, lowered.
+struct TileArrays {
+  std::vector<RegSeg> segs;
+  std::vector<TileRound> rounds;
+  std::vector<RegGate> gates;
+  std::vector<int> m2idx;
+  std::vector<double> mat2;
+};
+
+void build_tiles(hsv_batch* b, const std::vector<Lowered>& all, std::vector<BSeg>* segs, TileArrays* A) {
+  int tb = 0;
+  while ((1 << tb) < b->nt) ++tb;
+  const int RB = kTileBits - tb;
+  for (BSeg& S : *segs) {
+    S.tseg_first = 0;
+    S.tseg_count = -1;
+    if (S.bits != kTileBits || S.gate_count == 0 || b->nt < 128 || RB < 1 || RB > 5) continue;
+    std::vector<Lowered> Ls;
+    for (int r = S.gate_first; r < S.gate_first + S.gate_count; ++r) Ls.push_back(all[b->rec_gate[r]]);
+    TilePlan TP;
+    if (build_tile_plan(Ls, b->rec_inner[S.gate_first].data(), kTileBits, RB, &TP) != HSV_OK) continue;
+    const int rb = (int)A->rounds.size(), gb = (int)A->gates.size(), mb = (int)(A->mat2.size() / 32);
+    S.tseg_first = (int)A->segs.size();
+    S.tseg_count = (int)TP.segs.size();
+    for (RegSeg G : TP.segs) {
+      G.first += G.fast ? rb : gb;
+      A->segs.push_back(G);
+    }
+    for (TileRound Q : TP.rounds) {
+      for (int& g : Q.g)
+        if (g >= 0) g += gb;
+      Q.sfirst += gb;
+      A->rounds.push_back(Q);
+    }
+    for (size_t i = 0; i < TP.gates.size(); ++i) {
+      A->gates.push_back(TP.gates[i]);
+      A->m2idx.push_back(TP.m2idx[i] >= 0 ? TP.m2idx[i] + mb : -1);
+    }
+    A->mat2.insert(A->mat2.end(), TP.mat2.begin(), TP.mat2.end());
+  }
+}
+
+// Writes the arrays into b->d_tile (allocated here when absent), the gate
+// records' matrix pointers into the same buffer.
+int upload_tiles(hsv_batch* b, const TileArrays& A) {
+  const size_t seg_off = 0;
+  const size_t round_off = seg_off + A.segs.size() * sizeof(RegSeg);
+  const size_t gate_off = round_off + A.rounds.size() * sizeof(TileRound);
+  const size_t mat_off = gate_off + A.gates.size() * sizeof(RegGate);
+  const size_t bytes = mat_off + A.mat2.size() * sizeof(double);
+  if (!b->d_tile) {
+    BCHK(hipMalloc(&b->d_tile, std::max<size_t>(bytes, 16)));
+    b->tile_bytes = bytes;
+    b->tile_nseg = A.segs.size();
+    b->tile_nround = A.rounds.size();
+    b->tile_ngate = A.gates.size();
+    b->tile_nmat = A.mat2.size();
+  }
+  std::vector<unsigned char> host(std::max<size_t>(bytes, 16), 0);
+  auto* dev = static_cast<unsigned char*>(b->d_tile);
+  if (!A.segs.empty()) std::memcpy(host.data() + seg_off, A.segs.data(), A.segs.size() * sizeof(RegSeg));
+  if (!A.rounds.empty()) std::memcpy(host.data() + round_off, A.rounds.data(), A.rounds.size() * sizeof(TileRound));
+  RegGate* hg = reinterpret_cast<RegGate*>(host.data() + gate_off);
+  for (size_t i = 0; i < A.gates.size(); ++i) {
+    hg[i] = A.gates[i];
+    hg[i].m2 = A.m2idx[i] >= 0 ? reinterpret_cast<const double*>(dev + mat_off) + 32 * size_t(A.m2idx[i]) : nullptr;
+  }
+  if (!A.mat2.empty()) std::memcpy(host.data() + mat_off, A.mat2.data(), A.mat2.size() * sizeof(double));
+  BCHK(hipMemcpyAsync(b->d_tile, host.data(), bytes, hipMemcpyHostToDevice, b->stream));
+  BCHK(hipStreamSynchronize(b->stream));
+  return HSV_OK;
+}
+
+template <typename T>
+const T* tile_part(const hsv_batch* b, size_t off) {
+  return reinterpret_cast<const T*>(static_cast<const unsigned char*>(b->d_tile) + off);
+}
+
 template <typename R, int BS>
 int stage_attr(hsv_batch* b) {
   const size_t lds = (size_t(1) << b->L) * sizeof(typename VecT<R>::type);
@@ -373,9 +496,12 @@ template <typename R, int BS>
 void launch_stage(hsv_batch* b, const StageList& st, const int* ids, const uint64_t* cum) {
   const size_t lds = (size_t(1) << b->L) * sizeof(typename VecT<R>::type);
   const int grid = (int)std::min<uint64_t>(st.items, uint64_t(b->num_cu) * b->grid_per_cu);
+  const size_t round_off = b->tile_nseg * sizeof(RegSeg);
+  const size_t gate_off = round_off + b->tile_nround * sizeof(TileRound);
   hipLaunchKernelGGL((k_batch_stage<R, BS>), grid, BS, lds, b->stream,
                      static_cast<typename VecT<R>::type*>(b->state), b->d_segs, ids, cum, (int)st.seg_ids.size(),
-                     st.items, b->d_gates);
+                     st.items, b->d_gates, tile_part<RegSeg>(b, 0), tile_part<TileRound>(b, round_off),
+                     tile_part<RegGate>(b, gate_off));
 }
 
 template <typename R>
@@ -392,6 +518,16 @@ int build(hsv_batch* b) {
   if (int rc = plan(b, &recs)) {
     drop_plan(b);
     return rc;
+  }
+  {
+    std::vector<Lowered> all(b->gates.size());
+    for (size_t g = 0; g < b->gates.size(); ++g) {
+      std::string err;
+      if (int rc = lower(b->gates[g], &all[g], &err)) return bfail(rc, "gate " + std::to_string(g) + ": " + err);
+    }
+    TileArrays A;
+    build_tiles(b, all, &b->segs, &A);
+    if (int rc = upload_tiles(b, A)) return rc;
   }
   BCHK(hipMalloc(reinterpret_cast<void**>(&b->d_segs), std::max<size_t>(b->segs.size(), 1) * sizeof(BSeg)));
   BCHK(hipMalloc(reinterpret_cast<void**>(&b->d_gates), std::max<size_t>(recs.size(), 1) * sizeof(PartGate)));
@@ -558,6 +694,21 @@ int hsv_batch_set_gates(hsv_batch* b, const hsv_gate* gates) {
     while (b->gate_off[inst + 1] <= g) ++inst;
     recs[r] = part_gate_of(Ls[g], std::min(b->nq[inst], b->L), b->rec_inner[r].data());
   }
+  // the tile plans: the same layout of segments and rounds keeps the graph
+  // (new matrices in place); a different one (a phase gate that became or
+  // stopped being a sign flip) plans again at the next run
+  std::vector<BSeg> segs = b->segs;
+  TileArrays A;
+  build_tiles(b, Ls, &segs, &A);
+  bool same_tiles = A.segs.size() == b->tile_nseg && A.rounds.size() == b->tile_nround &&
+                    A.gates.size() == b->tile_ngate && A.mat2.size() == b->tile_nmat;
+  for (size_t i = 0; same_tiles && i < segs.size(); ++i)
+    same_tiles = segs[i].tseg_first == b->segs[i].tseg_first && segs[i].tseg_count == b->segs[i].tseg_count;
+  if (!same_tiles) {
+    drop_plan(b);
+    return HSV_OK;
+  }
+  if (int rc = upload_tiles(b, A)) return rc;
   if (!recs.empty())
     BCHK(hipMemcpyAsync(b->d_gates, recs.data(), recs.size() * sizeof(PartGate), hipMemcpyHostToDevice, b->stream));
   BCHK(hipStreamSynchronize(b->stream));

@@ -19,6 +19,7 @@
 #include "fused.hpp"
 #include "gates.hpp"
 #include "kernels.hpp"
+#include "tileplan.hpp"
 
 using namespace hsv;
 
@@ -456,22 +457,59 @@ int tile_part_bits(const hsv_state* s) {
   return s->tile_bits == def - 1 ? def - 1 : def;
 }
 
-// A part on the FP64 tile kernel (k_part_tile): the tile holds the part's
-// qubits padded with the lowest free slots (so slots 0-4 first) to 2^KT
-// amplitudes; the register plan is the register kernel's, over the tile's
-// bits, with KT - log2(NTH) register bits per thread.
-template <typename R, int KT, int NTH>
-int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t part_mask) {
-  constexpr int RB = KT - tile_log2(NTH);
-  const int n = s->n;
-  uint64_t tmask = part_mask;
-  for (int b = 0; b < n && __builtin_popcountll(tmask) < KT; ++b) tmask |= 1ull << b;
-  if (__builtin_popcountll(tmask) != KT) return fail(HSV_ERR_LIMIT, "part does not fit the tile kernel");
-  int inner_of[64], slot[KT];
-  for (int b = 0, r = 0; b < 64; ++b) {
-    inner_of[b] = (tmask >> b & 1) ? r : -1;
-    if (tmask >> b & 1) slot[r++] = b;
+}  // namespace
+
+// Runs of uncontrolled one-qubit gates on one qubit (no other gate on that
+// qubit in between) multiplied into one gate, later matrix times earlier:
+// the gates between them act on other qubits and commute with them. The
+// product is dense unless both are diagonal. Identities are dropped.
+std::vector<Lowered> hsv::fuse_one_qubit_runs(const std::vector<Lowered>& Ls) {
+  std::vector<Lowered> out;
+  int last[64];
+  for (int& x : last) x = -1;
+  for (const Lowered& L : Ls) {
+    if (L.kind == K_IDENT) continue;
+    if (L.nt == 1 && L.nc == 0) {
+      const int q = L.t[0];
+      if (last[q] < 0) {
+        last[q] = (int)out.size();
+        out.push_back(L);
+        continue;
+      }
+      Lowered& A = out[last[q]];
+      cd a[16], b[16];
+      dense_of(A, a);
+      dense_of(L, b);
+      const cd p0 = b[0] * a[0] + b[1] * a[2], p1 = b[0] * a[1] + b[1] * a[3];
+      const cd p2 = b[2] * a[0] + b[3] * a[2], p3 = b[2] * a[1] + b[3] * a[3];
+      const bool diag = (A.kind == K_PHASE1 || A.kind == K_DIAG1) && (L.kind == K_PHASE1 || L.kind == K_DIAG1);
+      if (diag) {
+        A.kind = K_DIAG1;
+        A.m[0] = p0;
+        A.m[1] = p3;
+      } else {
+        A.kind = K_DENSE1;
+        A.m[0] = p0;
+        A.m[1] = p1;
+        A.m[2] = p2;
+        A.m[3] = p3;
+      }
+      continue;
+    }
+    for (int i = 0; i < L.nt; ++i) last[L.t[i]] = -1;
+    for (int i = 0; i < L.nc; ++i) last[L.c[i]] = -1;
+    out.push_back(L);
   }
+  return out;
+}
+
+// The FP64 tile kernel's plan of one part (shared with the circuit batch):
+// the part's one-qubit runs fused (fuse_one_qubit_runs), then the register
+// kernel's layout segments over the tile's KT bits with RB register bits;
+// fast segments (only uncontrolled one-qubit dense gates and sign flips)
+// become rounds. Indices are relative to the plan.
+int hsv::build_tile_plan(const std::vector<Lowered>& Ls_in, const int* inner_of, int KT, int RB, TilePlan* T) {
+  const std::vector<Lowered> Ls = fuse_one_qubit_runs(Ls_in);
   RegPlan P;
   plan_register_part(Ls, inner_of, KT, RB, &P, 0);
   // fast segments (only uncontrolled one-qubit dense gates and sign flips)
@@ -479,9 +517,9 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
   // sign flips; a dense gate after a flip, or on a slot the round already
   // has, starts the next round (so no gate moves past one it does not
   // commute with)
-  std::vector<TileRound> rounds;
+  std::vector<TileRound>& rounds = T->rounds;
   for (RegSeg& S : P.segs) {
-    if (__builtin_popcount(S.regmask) != RB) return fail(HSV_ERR_LIMIT, "tile layout without its register bits");
+    if (__builtin_popcount(S.regmask) != RB) return HSV_ERR_LIMIT;
     S.fast = 1;
     for (int i = S.first; i < S.first + S.count; ++i) {
       const RegGate& G = P.gates[i];
@@ -523,6 +561,40 @@ int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t pa
     S.first = r0;
     S.count = (int)rounds.size() - r0;
   }
+  T->segs = std::move(P.segs);
+  T->gates = std::move(P.gates);
+  T->m2idx = std::move(P.m2idx);
+  T->mat2 = std::move(P.mat2);
+  return HSV_OK;
+}
+
+namespace {
+
+// A part on the FP64 tile kernel (k_part_tile): the tile holds the part's
+// qubits padded with the lowest free slots (so slots 0-4 first) to 2^KT
+// amplitudes; the register plan is the register kernel's, over the tile's
+// bits, with KT - log2(NTH) register bits per thread.
+template <typename R, int KT, int NTH>
+int launch_tile_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t part_mask) {
+  constexpr int RB = KT - tile_log2(NTH);
+  const int n = s->n;
+  uint64_t tmask = part_mask;
+  for (int b = 0; b < n && __builtin_popcountll(tmask) < KT; ++b) tmask |= 1ull << b;
+  if (__builtin_popcountll(tmask) != KT) return fail(HSV_ERR_LIMIT, "part does not fit the tile kernel");
+  int inner_of[64], slot[KT];
+  for (int b = 0, r = 0; b < 64; ++b) {
+    inner_of[b] = (tmask >> b & 1) ? r : -1;
+    if (tmask >> b & 1) slot[r++] = b;
+  }
+  TilePlan TP;
+  if (build_tile_plan(Ls, inner_of, KT, RB, &TP) != HSV_OK)
+    return fail(HSV_ERR_LIMIT, "tile layout without its register bits");
+  const std::vector<TileRound>& rounds = TP.rounds;
+  RegPlan P;
+  P.segs = std::move(TP.segs);
+  P.gates = std::move(TP.gates);
+  P.m2idx = std::move(TP.m2idx);
+  P.mat2 = std::move(TP.mat2);
   TileTabs T{};
   for (int v = 0; v < 256; ++v)
     for (int j = 0; j < 8; ++j) {

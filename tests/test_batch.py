@@ -46,6 +46,29 @@ def test_batch_matches_oracle(orc, seed):
             assert np.abs(b.state(i) - oracle_state(orc, n, g)).max() <= 1e-12, i
 
 
+@pytest.mark.parametrize("threads", [128, 256, 512])
+def test_batch_tile_plans(orc, threads):
+    """Chunks of 12 qubits run the FP64 tile kernel's register segments (5,
+    4 or 3 register bits per thread at 128, 256 or 512 threads): random
+    circuits of every gate type with 0-2 controls, 12-15 qubits (one to
+    several segments per instance), each instance against the oracle; then
+    new parameters of the same structure through the same graph."""
+    insts = random_instances(40 + threads, 10, (12, 15), ngates=(40, 100))
+    with CircuitBatch(insts) as b:
+        b.tune("chunk_bits", 12)
+        b.tune("threads", threads)
+        b.run()
+        for i, (n, g) in enumerate(insts):
+            assert np.abs(b.state(i) - oracle_state(orc, n, g)).max() <= 1e-12, i
+        rng = np.random.default_rng(threads)
+        new = [(n, [Gate(x.name, x.targets, x.controls, tuple(p * rng.uniform(0.5, 1.5) if isinstance(p, float) else p
+                                                                 for p in x.params)) for x in g]) for n, g in insts]
+        b.set_gates([g for _, g in new])
+        b.run()
+        for i, (n, g) in enumerate(new):
+            assert np.abs(b.state(i) - oracle_state(orc, n, g)).max() <= 1e-12, i
+
+
 @pytest.mark.parametrize("chunk_bits", [6, 8])
 def test_batch_wide_instances_many_stages(orc, chunk_bits):
     """Instances of 12-18 qubits on small chunks: many segments per instance,
