@@ -452,8 +452,9 @@ int launch_reg_part_t(hsv_state* s, const std::vector<Lowered>& Ls, uint64_t par
 // Tile width of the FP64 tile part kernel: by default 64 KB of LDS per
 // workgroup of 512 threads (two workgroups per CU); tile_bits one less
 // gives 32 KB tiles of 256 threads (four per CU).
+int tile_part_default(const hsv_state* s) { return s->dtype == HSV_C128 ? 12 : 13; }
 int tile_part_bits(const hsv_state* s) {
-  const int def = s->dtype == HSV_C128 ? 12 : 13;
+  const int def = tile_part_default(s);
   return s->tile_bits == def - 1 ? def - 1 : def;
 }
 
@@ -1418,7 +1419,8 @@ int hsv_tune(hsv_state* s, const char* key, int value) {
   if (k == "reg_waves" && value >= 1 && value <= 64) s->reg_waves = value;
   else if (k == "part_kernel" && value >= 0 && value <= 2) s->part_kernel = value;
   else if (k == "reg_bits" && value >= 0 && value <= 5) s->reg_bits = value;
-  else if (k == "tile_bits" && value >= 0 && value <= 13) s->tile_bits = value;
+  else if (k == "tile_bits" && (value == 0 || value == tile_part_default(s) || value == tile_part_default(s) - 1))
+    s->tile_bits = value;
   else if (k == "low_slots" && value >= 0 && value <= 6) s->low_slots = value;
   else if (k == "io_bits" && value >= 0 && value <= 6) s->io_bits = value;
   else if (k == "fused_variant" && value >= 0 && value <= 2) s->bf.variant = value;

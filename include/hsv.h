@@ -242,11 +242,11 @@ int hsv_fused_unitary(int num_qubits, const hsv_gate* gates, int num_gates, int*
  *                 registers segment by segment), else as 1;
  *                 1 = register-blocked part kernel for parts of >= 8 qubits,
  *                 0 = LDS kernel for every part;
- *   "tile_bits"   width of the FP64 tile kernel's tile: 0 (the default) =
- *                 12 qubits (13 in complex64), 512 threads, two workgroups
- *                 per CU; 11 (12 in complex64) = 256 threads, 32 KB of LDS,
+ *   "tile_bits"   width of the FP64 tile kernel's tile: 0 or 12 (13 in
+ *                 complex64, the default) = 512 threads, two workgroups per
+ *                 CU; 11 (12 in complex64) = 256 threads, 32 KB of LDS,
  *                 four workgroups per CU (faster per pass, but parts of one
- *                 qubit less);
+ *                 qubit less); other values are refused;
  *   "reg_bits"    preferred values per thread of the register kernel, as
  *                 log2 (2..5; 0 = default for the dtype);
  *   "reg_waves"   persistent workgroups per CU and occupancy slot (1..64);

@@ -69,6 +69,18 @@ def test_batch_tile_plans(orc, threads):
             assert np.abs(b.state(i) - oracle_state(orc, n, g)).max() <= 1e-12, i
 
 
+def test_batch_tile_plans_complex64(orc):
+    """The tile kernel's register segments on 12-qubit chunks of complex64
+    states (FP64 arithmetic on FP32 storage), against the oracle at 1e-5."""
+    insts = random_instances(77, 10, (12, 15), ngates=(40, 100))
+    with CircuitBatch(insts, "complex64") as b:
+        b.tune("chunk_bits", 12)
+        b.tune("threads", 512)
+        b.run()
+        for i, (n, g) in enumerate(insts):
+            assert np.abs(b.state(i) - oracle_state(orc, n, g)).max() <= 1e-5, i
+
+
 @pytest.mark.parametrize("chunk_bits", [6, 8])
 def test_batch_wide_instances_many_stages(orc, chunk_bits):
     """Instances of 12-18 qubits on small chunks: many segments per instance,

@@ -771,6 +771,8 @@ def test_tile_part_kernel_256_threads(orc, dtype, n):
     orc.apply_gates(want, gates)
     with capi.State(n, dtype) as s:
         s.tune("part_kernel", 2)
+        with pytest.raises(capi.HsvError):
+            s.tune("tile_bits", kt - 1)   # only the default width and one less
         s.tune("tile_bits", kt)
         s.write(a)
         for p in parts:
