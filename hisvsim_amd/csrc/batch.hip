@@ -12,9 +12,10 @@
 // segment of every instance that has one: a persistent kernel walks the
 // flattened (instance, chunk) list of the stage, gathers each chunk (the
 // segment's qubits plus the instance's lowest other qubits) into LDS,
-// applies the segment's gates there and scatters it back. A chunk of 12
-// qubits runs the FP64 tile kernel's plan (gates in registers, one LDS
-// exchange per register layout); smaller ones sweep LDS once per gate. The stage launches (after a fill kernel that sets
+// applies the segment's gates there and scatters it back. A chunk of 8-12
+// qubits (13 in complex64) runs the FP64 tile kernel's plan (gates in
+// registers, one LDS exchange per register layout) when the thread count
+// leaves it 2..5 register bits; other chunks sweep LDS once per gate. The stage launches (after a fill kernel that sets
 // every instance to |0...0>) are captured once into a hipGraph; gate
 // matrices live in device memory, so new parameters for the same gate
 // structure replay the same graph.
@@ -57,15 +58,20 @@ struct BSeg {
   int bits;             // popcount(chunk_mask)
   int gate_first;       // first PartGate record
   int gate_count;
-  int tseg_first;       // chunks of kTileBits qubits: first register segment of its tile plan
+  int tseg_first;       // chunks with a tile plan: its first register segment
   int tseg_count;       // -1: the gates sweep the chunk in LDS one by one
   int pad;
 };
 
-// Chunks of this many qubits run the FP64 tile kernel's register segments
-// (tile_segments: each thread holds 2^(kTileBits - log2 threads) amplitudes,
-// one LDS exchange per segment instead of one LDS sweep per gate).
-constexpr int kTileBits = 12;
+// A chunk of KT qubits runs the FP64 tile kernel's register segments
+// (tile_segments: each thread holds 2^RB amplitudes, RB = KT - log2 threads,
+// one LDS exchange per segment instead of one LDS sweep per gate) when RB is
+// 2..5 (a two-target gate needs both targets in registers); other chunks
+// sweep LDS once per gate.
+template <typename R, int KT, int BS>
+constexpr bool tile_chunk_ok() {
+  return KT - tile_log2(BS) >= 2 && KT - tile_log2(BS) <= 5 && (sizeof(R) == 4 || KT <= 12);
+}
 
 // Stage work list: the segments taking part (seg_ids) and the first
 // flattened item of each (cum).
@@ -138,7 +144,7 @@ __global__ __launch_bounds__(BS) void k_batch_stage(typename VecT<R>::type* __re
     const uint64_t base = S.amp_off + pdep64(it - cum[k], S.outer_mask);
     const uint32_t nin = 1u << S.bits;
     if constexpr (BS >= 128) {
-      if (S.tseg_count >= 0) {  // uniform: a 2^kTileBits chunk with a tile plan
+      if (S.tseg_count >= 0) {  // uniform: a chunk with a tile plan
         cplx<R>* ys = reinterpret_cast<cplx<R>*>(smem_raw);
         for (uint32_t j = threadIdx.x; j < nin; j += BS) {
           const V a = st[base + tlo[j & 63] + thi[j >> 6]];
@@ -147,7 +153,17 @@ __global__ __launch_bounds__(BS) void k_batch_stage(typename VecT<R>::type* __re
           c.y = a.y;
           ys[tile_sw(j)] = c;
         }
-        tile_segments<R, kTileBits, BS>(ys, tsegs + S.tseg_first, S.tseg_count, trounds, tgates);
+        const RegSeg* ts = tsegs + S.tseg_first;
+        const int ns = S.tseg_count;
+        switch (S.bits) {  // the host plans only the widths tile_chunk_ok accepts
+          case 8: if constexpr (tile_chunk_ok<R, 8, BS>()) tile_segments<R, 8, BS>(ys, ts, ns, trounds, tgates); break;
+          case 9: if constexpr (tile_chunk_ok<R, 9, BS>()) tile_segments<R, 9, BS>(ys, ts, ns, trounds, tgates); break;
+          case 10: if constexpr (tile_chunk_ok<R, 10, BS>()) tile_segments<R, 10, BS>(ys, ts, ns, trounds, tgates); break;
+          case 11: if constexpr (tile_chunk_ok<R, 11, BS>()) tile_segments<R, 11, BS>(ys, ts, ns, trounds, tgates); break;
+          case 12: if constexpr (tile_chunk_ok<R, 12, BS>()) tile_segments<R, 12, BS>(ys, ts, ns, trounds, tgates); break;
+          case 13: if constexpr (tile_chunk_ok<R, 13, BS>()) tile_segments<R, 13, BS>(ys, ts, ns, trounds, tgates); break;
+          default: break;
+        }
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < nin; j += BS) {
           const cplx<R> c = ys[tile_sw(j)];
@@ -269,7 +285,7 @@ struct hsv_batch {
   std::vector<StageList> stages;
   BSeg* d_segs = nullptr;
   PartGate* d_gates = nullptr;
-  // tile plans of the kTileBits-qubit segments, one device buffer:
+  // tile plans of the segments that have one, one device buffer:
   // register segments | rounds | gate records | dense two-qubit matrices
   void* d_tile = nullptr;
   size_t tile_bytes = 0;
@@ -395,10 +411,11 @@ int plan(hsv_batch* b, std::vector<PartGate>* recs) {
   return HSV_OK;
 }
 
-// The tile plans of every segment whose chunk holds kTileBits qubits (with
-// at least 128 threads per workgroup: at most 5 register bits), as one set
-// of arrays with absolute indices; sets each segment's tseg_first/count
-// (-1: LDS sweeps). all[g]: gate g of the batch, lowered.
+// The tile plans of every segment whose chunk width k_batch_stage has a
+// tile form for (8..12 qubits, 13 in complex64, 2..5 register bits at the
+// batch's thread count, at least 128 threads), as one set of arrays with
+// absolute indices; sets each segment's tseg_first/count (-1: LDS sweeps).
+// all[g]: gate g of the batch, lowered.
 struct TileArrays {
   std::vector<RegSeg> segs;
   std::vector<TileRound> rounds;
@@ -410,15 +427,17 @@ struct TileArrays {
 void build_tiles(hsv_batch* b, const std::vector<Lowered>& all, std::vector<BSeg>* segs, TileArrays* A) {
   int tb = 0;
   while ((1 << tb) < b->nt) ++tb;
-  const int RB = kTileBits - tb;
   for (BSeg& S : *segs) {
     S.tseg_first = 0;
     S.tseg_count = -1;
-    if (S.bits != kTileBits || S.gate_count == 0 || b->nt < 128 || RB < 1 || RB > 5) continue;
+    const int RB = S.bits - tb;
+    // the widths k_batch_stage instantiates (tile_chunk_ok)
+    const bool ok = S.bits >= 8 && S.bits <= (b->dtype == HSV_C128 ? 12 : 13) && RB >= 2 && RB <= 5 && b->nt >= 128;
+    if (!ok || S.gate_count == 0) continue;
     std::vector<Lowered> Ls;
     for (int r = S.gate_first; r < S.gate_first + S.gate_count; ++r) Ls.push_back(all[b->rec_gate[r]]);
     TilePlan TP;
-    if (build_tile_plan(Ls, b->rec_inner[S.gate_first].data(), kTileBits, RB, &TP) != HSV_OK) continue;
+    if (build_tile_plan(Ls, b->rec_inner[S.gate_first].data(), S.bits, RB, &TP) != HSV_OK) continue;
     const int rb = (int)A->rounds.size(), gb = (int)A->gates.size(), mb = (int)(A->mat2.size() / 32);
     S.tseg_first = (int)A->segs.size();
     S.tseg_count = (int)TP.segs.size();

@@ -510,6 +510,7 @@ std::vector<Lowered> hsv::fuse_one_qubit_runs(const std::vector<Lowered>& Ls) {
 // fast segments (only uncontrolled one-qubit dense gates and sign flips)
 // become rounds. Indices are relative to the plan.
 int hsv::build_tile_plan(const std::vector<Lowered>& Ls_in, const int* inner_of, int KT, int RB, TilePlan* T) {
+  if (RB < 2 || RB > 5 || RB > KT) return HSV_ERR_LIMIT;  // the planner places two-target gates in registers
   const std::vector<Lowered> Ls = fuse_one_qubit_runs(Ls_in);
   RegPlan P;
   plan_register_part(Ls, inner_of, KT, RB, &P, 0);

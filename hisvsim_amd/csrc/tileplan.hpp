@@ -18,7 +18,8 @@ struct TilePlan {
 };
 
 // Ls: the part's gates; inner_of[q]: tile bit of qubit q (-1 outside the
-// tile); KT tile bits, RB register bits per thread. Runs of uncontrolled
+// tile); KT tile bits, RB >= 2 register bits per thread (a gate with two
+// non-diagonal targets needs both in registers). Runs of uncontrolled
 // one-qubit gates on one qubit are first multiplied into one gate.
 // HSV_ERR_LIMIT when a segment's layout does not hold RB register bits.
 int build_tile_plan(const std::vector<Lowered>& Ls, const int* inner_of, int KT, int RB, TilePlan* T);

@@ -46,16 +46,19 @@ def test_batch_matches_oracle(orc, seed):
             assert np.abs(b.state(i) - oracle_state(orc, n, g)).max() <= 1e-12, i
 
 
-@pytest.mark.parametrize("threads", [128, 256, 512])
-def test_batch_tile_plans(orc, threads):
-    """Chunks of 12 qubits run the FP64 tile kernel's register segments (5,
-    4 or 3 register bits per thread at 128, 256 or 512 threads): random
-    circuits of every gate type with 0-2 controls, 12-15 qubits (one to
-    several segments per instance), each instance against the oracle; then
-    new parameters of the same structure through the same graph."""
-    insts = random_instances(40 + threads, 10, (12, 15), ngates=(40, 100))
+@pytest.mark.parametrize("chunk_bits,threads", [(12, 128), (12, 256), (12, 512), (11, 256), (11, 512), (10, 256)])
+def test_batch_tile_plans(orc, chunk_bits, threads):
+    """Chunks run the FP64 tile kernel's register segments when the chunk
+    width less log2(threads) leaves 2..5 register bits per thread (at 12
+    qubits: 5, 4, 3 at 128, 256, 512 threads), else LDS sweeps (a 10-qubit
+    instance at 512 threads): random circuits of every gate type with 0-2
+    controls, 9-15 qubits (one to several segments per instance, chunks of
+    the instance's width below the chunk size), each instance against the
+    oracle; then new parameters of the same structure through the same
+    graph."""
+    insts = random_instances(40 + threads + chunk_bits, 12, (9, 15), ngates=(40, 100))
     with CircuitBatch(insts) as b:
-        b.tune("chunk_bits", 12)
+        b.tune("chunk_bits", chunk_bits)
         b.tune("threads", threads)
         b.run()
         for i, (n, g) in enumerate(insts):
