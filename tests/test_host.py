@@ -343,3 +343,21 @@ def test_check_parts_detects_reordering():
     gates = [Gate("H", (0,)), Gate("X", (1,), (0,)), Gate("H", (0,))]
     with pytest.raises(ValueError):
         check_parts(gates, [[gates[2]], [gates[0], gates[1]]])
+
+
+def test_bench_reads_the_measured_traffic_of_its_default_workloads():
+    """bench.py's roofline.traffic comes from the committed PMC summary
+    (profiles/traffic.json, read at run time, so the folder must travel with
+    the tree): the default chain, the three-piece chain and the FP64 part
+    path each have a row, within 1% of their algorithmic bytes."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    algo = 2 * 16 * 2 ** 33
+    for kernel, mode, k in [("k_part_bf", "fusedbf16", 13), ("k_part_bf", "fusedbf16x3", 13), ("k_part_tile", "part", 12)]:
+        t = bench.measured_traffic(kernel, 33, mode, k)
+        assert t is not None and abs(t / algo - 1) <= 0.01, (kernel, mode, t)
+    with open(os.path.join(ROOT, ".gpurunignore")) as f:
+        ignored = [line.strip().rstrip("/") for line in f]
+    assert "./profiles" not in ignored and "profiles" not in ignored
