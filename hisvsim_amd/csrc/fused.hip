@@ -235,6 +235,23 @@ __device__ __forceinline__ void split_bf8_3(const float2 (&v)[8], bf8* rh, bf8* 
   *il = __builtin_bit_cast(bf8, f);
 }
 
+// Column signs of the three-piece split. column_hash: murmur3's 32-bit
+// finaliser of (tile << 8 | column) (columns < 256), once per tile;
+// column_sign: 0 or the FP32 sign bit for one step, the top bit of
+// (hash ^ step * 0x85EBCA77) * 0x9E3779B1.
+__device__ __forceinline__ uint32_t column_hash(uint64_t tile, int col) {
+  uint32_t k = uint32_t(tile) << 8 | uint32_t(col);
+  k ^= k >> 16;
+  k *= 0x85EBCA6Bu;
+  k ^= k >> 13;
+  k *= 0xC2B2AE35u;
+  k ^= k >> 16;
+  return k;
+}
+__device__ __forceinline__ uint32_t column_sign(uint32_t hash, int step) {
+  return ((hash ^ (uint32_t(step) * 0x85EBCA77u)) * 0x9E3779B1u) & 0x80000000u;
+}
+
 struct BfArgs {
   uint64_t nblocks;     // column blocks of 16 columns
   uint64_t outer_mask;  // outer bits above the 4 column bits
@@ -624,6 +641,12 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
 #pragma unroll
       for (int j = 0; j < PER; ++j) raw[j] = ld_stream<R>(&a[nbase + a_lo + ahi[wave + W * j]]);
     }
+    // three pieces: each of this lane's columns keeps its hash for the tile
+    uint32_t colh[NP == 3 ? CBW : 1];
+    if constexpr (NP == 3) {
+#pragma unroll
+      for (int w = 0; w < CBW; ++w) colh[w] = column_hash(t0 + it, 16 * (wave * CBW + w) + cc);
+    }
     for (int s = 0; s < nsteps; ++s) {
       const bf8* As;
       bf8 Af[2][4];
@@ -676,14 +699,26 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
         // first into the FP32 accumulator; one column block at a time (each
         // wave owns its columns, so a block is read, multiplied and written
         // before the next without a hazard), which keeps the eighteen BF16
-        // operands of a block and its accumulators within 128 VGPRs
+        // operands of a block and its accumulators within 128 VGPRs.
+        // The matrix cores' FP32 accumulation rounds with a small bias of
+        // one direction: a pass's error has a component that is the same in
+        // every tile and for every input, which later Hadamards (a QFT's)
+        // add up onto a few amplitudes. Each column's 32 values are
+        // negated before the products and the results negated back, by a
+        // sign hashed from (tile, column, step): exact, and bit-identical
+        // under unbiased rounding, while the bias of each column then
+        // enters with a pseudo-random sign and no longer adds up.
 #pragma unroll
         for (int w = 0; w < CBW; ++w) {
+          const uint32_t fl = column_sign(colh[w], s);
           bf8 xrh, xrm, xrl, xih, xim, xil;
           {
             float2 v[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = ys[roff(e) ^ tc[w]];
+            for (int e = 0; e < 8; ++e) {
+              const float2 y = ys[roff(e) ^ tc[w]];
+              v[e] = make_float2(flip_sign(y.x, fl), flip_sign(y.y, fl));
+            }
             split_bf8_3(v, &xrh, &xrm, &xrl, &xih, &xim, &xil);
           }
 #pragma unroll
@@ -717,7 +752,8 @@ __global__ __launch_bounds__(part_bf_threads(KT, STREAM_U)) __attribute__((amdgp
             accJ = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uih, xrh, accJ, 0, 0, 0);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-              ys[roff(4 * rb + i) ^ tc[w]] = make_float2(accR[i] - accT[i], accI[i] + accJ[i]);
+              ys[roff(4 * rb + i) ^ tc[w]] =
+                  make_float2(flip_sign(accR[i] - accT[i], fl), flip_sign(accI[i] + accJ[i], fl));
           }
         }
         continue;
