@@ -389,6 +389,30 @@ def test_fused_chain_round_trip_full_size(n, dtype):
     assert abs(norm2 - 1) <= (1e-12 if dtype == "complex128" else 1e-5)
 
 
+@pytest.mark.parametrize("n,dtype", [(33, "complex128"), (34, "complex64")])
+def test_three_piece_chain_round_trip_full_size(n, dtype):
+    """The same round trip on the three-piece split at the benchmark sizes
+    (2^20 and 2^21 tiles of 13 qubits, so the kernel's column signs hash
+    its largest tile indices): unit norm, and the probability mass left
+    outside |0...0> (the FP64 norm of the state with amplitude 0 cleared,
+    which complex64's rounding of a0 itself cannot hide) <= 1e-11
+    (measured 2.4e-13 at 33 qubits complex128, 2.5e-13 at 34 complex64)."""
+    from conftest import layered_inverse
+    from hisvsim_amd.partition import chain_parts
+    gates = random_layered(n, 12, seed=n)
+    parts, lows = chain_parts(gates + layered_inverse(gates), 13, beam=(4, 4))
+    with capi.State(n, dtype) as s:
+        s.apply_fused_chain(parts, lows, "bf16x3", renormalize=True)
+        norm2 = s.norm2()
+        a0 = s.read(0, 1)
+        s.write(np.zeros_like(a0), 0)
+        rest = s.norm2()
+    print(f"three-piece round trip n={n} {dtype}: mass outside |0> {rest:.2e}, "
+          f"|norm2 - 1| = {abs(norm2 - 1):.2e}", flush=True)
+    assert rest <= 1e-11, (rest, n, dtype)
+    assert abs(norm2 - 1) <= (1e-12 if dtype == "complex128" else 1e-5)
+
+
 def test_fused_chain_renormalize_zero_state():
     """A zero state stays zero (no error) under the renormalising chain."""
     from hisvsim_amd.partition import chain_parts
