@@ -82,3 +82,21 @@ def test_column_signs_spread_a_directed_accumulation_bias_on_qft():
     assert down_ratio >= 15
     assert flip_ratio <= 8
     assert flip_max <= down_max / 4
+
+
+def test_model_hash_matches_the_kernel_source():
+    """The model's column hash and sign use the constants of column_hash and
+    column_sign in fused.hip, so the model studies the kernel's signs."""
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                            "hisvsim_amd", "csrc", "fused.hip")).read()
+    body_hash = src[src.index("uint32_t column_hash("):]
+    body_hash = body_hash[:body_hash.index("\n}")]
+    body_sign = src[src.index("uint32_t column_sign("):]
+    body_sign = body_sign[:body_sign.index("\n}")]
+    consts = lambda body: [int(c, 16) for c in re.findall(r"0x([0-9A-Fa-f]+)u", body)]
+    # the constants bf16x3_model.column_hash / column_sign use
+    assert consts(body_hash) == [0x85EBCA6B, 0xC2B2AE35]
+    assert consts(body_sign) == [0x85EBCA77, 0x9E3779B1, 0x80000000]
+    assert "<< 8" in body_hash
