@@ -96,7 +96,11 @@ def test_model_hash_matches_the_kernel_source():
     body_sign = src[src.index("uint32_t column_sign("):]
     body_sign = body_sign[:body_sign.index("\n}")]
     consts = lambda body: [int(c, 16) for c in re.findall(r"0x([0-9A-Fa-f]+)u", body)]
-    # the constants bf16x3_model.column_hash / column_sign use
     assert consts(body_hash) == [0x85EBCA6B, 0xC2B2AE35]
     assert consts(body_sign) == [0x85EBCA77, 0x9E3779B1, 0x80000000]
     assert "<< 8" in body_hash
+    # and the model's
+    import inspect
+    model_consts = lambda fn: [int(c, 16) for c in re.findall(r"0x([0-9A-Fa-f]+)", inspect.getsource(fn))]
+    assert set(model_consts(M.column_hash)) == {0x85EBCA6B, 0xC2B2AE35}
+    assert set(model_consts(M.column_sign)) - {0xFFFFFFFF} == {0x85EBCA77, 0x9E3779B1}
